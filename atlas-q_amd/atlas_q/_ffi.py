@@ -1,0 +1,79 @@
+"""ctypes binding of the C ABI in ``include/atlasq.h``.
+
+The shared library ``libatlasq.so`` sits next to this file (built in-tree by
+``atlas-q_amd/build.py``).  Loading it needs only the ROCm runtime, not a GPU,
+so symbol checks run anywhere; every compute call needs a HIP device and fails
+loudly otherwise -- there is no CPU fallback anywhere in this package.
+"""
+from __future__ import annotations
+
+import ctypes
+import re
+from pathlib import Path
+
+LIB_PATH = Path(__file__).with_name("libatlasq.so")
+HEADER = Path(__file__).resolve().parents[2] / "include" / "atlasq.h"
+
+AQ_OK = 0
+AQ_C64, AQ_C128 = 0, 1
+AQ_PREC_NATIVE, AQ_PREC_BF16X3 = 0, 1
+AQ_RANK_REF = 0
+
+c_i64 = ctypes.c_int64
+c_int = ctypes.c_int
+c_vp = ctypes.c_void_p
+c_dp = ctypes.POINTER(ctypes.c_double)
+
+
+class RankResult(ctypes.Structure):
+    _fields_ = [("k", ctypes.c_int32), ("flags", ctypes.c_int32),
+                ("eps_local", ctypes.c_double), ("entropy", ctypes.c_double),
+                ("condS", ctypes.c_double), ("total_weight", ctypes.c_double),
+                ("kept_weight", ctypes.c_double)]
+
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "aq_version": (c_int, []),
+    "aq_last_error": (ctypes.c_char_p, []),
+    "aq_theta": (c_int, [c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_i64, c_vp]),
+    "aq_rank_select": (c_int, [c_int, c_vp, c_i64, ctypes.c_double, c_i64, c_int, c_vp, c_vp]),
+    "aq_rebuild": (c_int, [c_int, c_vp, c_i64, c_i64, c_vp, c_vp, c_i64, c_i64, c_i64, c_i64,
+                           c_i64, c_vp, c_vp, c_vp]),
+    "aq_apply_1q": (c_int, [c_int, c_vp, c_vp, c_i64, c_i64, c_vp, c_vp]),
+    "aq_cgemm": (c_int, [c_int, c_int, c_i64, c_i64, c_i64, c_vp, c_i64, c_i64, c_int,
+                         c_vp, c_i64, c_i64, c_int, c_vp, c_i64, c_i64, c_vp]),
+    "aq_sample_workspace_bytes": (c_i64, [c_int, c_i64, c_vp, c_i64]),
+    "aq_sample": (c_int, [c_int, c_int, c_int, c_i64, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp,
+                          c_i64, c_vp]),
+}
+
+_lib = None
+
+
+def header_symbols() -> list:
+    """Entry points declared in include/atlasq.h."""
+    txt = HEADER.read_text()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?[\w\*\s]+?\b(aq_\w+)\s*\(", txt, re.M)))
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(
+                f"atlas_q: HIP library {LIB_PATH} is missing; build it with "
+                "`python atlas-q_amd/build.py` (no CPU fallback exists)")
+        L = ctypes.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status: int, what: str) -> None:
+    if status != AQ_OK:
+        msg = lib().aq_last_error().decode(errors="replace")
+        raise RuntimeError(f"{what} failed (status {status}): {msg}")

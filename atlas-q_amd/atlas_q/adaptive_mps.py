@@ -1,0 +1,303 @@
+"""Adaptive MPS on MI355X (API of the reference's ``src/atlas_q/adaptive_mps.py``).
+
+``AdaptiveMPS.apply_two_site_gate`` keeps the reference's semantics
+(adaptive_mps.py:182-308): theta in the "U" convention, thin SVD, the rank
+rule of truncation.py with the per-bond cap and the global memory budget,
+promotion to complex128 when cond(S) exceeds the policy, S absorbed into the
+left core, no renormalisation, statistics per update.  The arithmetic runs in
+the HIP kernels of libatlasq.so: theta (MFMA), rank rule, core rebuild; the
+plain method's SVD is rocSOLVER through ``torch.linalg.svd``.
+
+Extensions (keyword-only, defaults reproduce the reference):
+* ``svd_method``: "plain" (full thin SVD, the reference's path) or one of the
+  truncated-SVD methods of ``atlas_q.fastsvd`` ("gram", "rsvd", "learned").
+* ``theta_precision``: "native" (FP32 MFMA for complex64) or "bf16x3".
+"""
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass
+from typing import Dict, List, Optional, Union
+
+import numpy as np
+import torch
+
+from . import kernels as K
+from .diagnostics import MPSStatistics
+from .linalg_robust import robust_qr, robust_svd
+from .mps_pytorch import (MatrixProductStatePyTorch, _CNOT, _CY, _CZ, _H, _S, _SDG, _SWAP,
+                          _T, _TDG, _X, _Y, _Z, rx_matrix, ry_matrix, rz_matrix)
+from .truncation import check_entropy_sanity, choose_rank_from_sigma
+
+# The gate convention of the reference's installed-triton path
+# (mps_complex.py:363): row index of U = output pair, left qubit = high bit.
+GATE_CONVENTION = "U"
+fused_two_qubit_gate = K.theta
+
+
+@dataclass
+class DTypePolicy:
+    """adaptive_mps.py:43-48."""
+
+    default: torch.dtype = torch.complex64
+    promote_if_cond_gt: float = 1e6
+
+
+def _gate4(U4) -> np.ndarray:
+    shape = tuple(U4.shape)
+    if shape not in ((2, 2, 2, 2), (4, 4)):
+        raise ValueError(f"U4 must be (2,2,2,2) or (4,4), got {U4.shape}")
+    return K.gate_to_host(U4, (4, 4))
+
+
+class AdaptiveMPS(MatrixProductStatePyTorch):
+    """adaptive_mps.py:51-623 on the HIP device."""
+
+    def __init__(self, num_qubits: int, bond_dim: int = 8, *, eps_bond: float = 1e-6,
+                 chi_max_per_bond: Optional[Union[int, List[int]]] = 256,
+                 budget_global_mb: Optional[float] = None,
+                 dtype_policy: DTypePolicy = DTypePolicy(), device: str = "cuda",
+                 dtype: Optional[torch.dtype] = None, svd_method: str = "plain",
+                 theta_precision: str = "native", rank_predictor=None):
+        if dtype is None:
+            dtype = dtype_policy.default
+        super().__init__(num_qubits, bond_dim, device, dtype)
+        self.eps_bond = eps_bond
+        self.dtype_policy = dtype_policy
+        self.budget_global_mb = budget_global_mb
+        if isinstance(chi_max_per_bond, int):
+            self.chi_max_per_bond = [chi_max_per_bond] * (num_qubits - 1)
+        else:
+            assert len(chi_max_per_bond) == num_qubits - 1, \
+                f"chi_max_per_bond must have length {num_qubits - 1}"
+            self.chi_max_per_bond = list(chi_max_per_bond)
+        self.bond_dims = [bond_dim] * (num_qubits - 1)
+        self.statistics = MPSStatistics()
+        self._operation_counter = 0
+        self.svd_method = svd_method
+        self.theta_precision = theta_precision
+        self.rank_predictor = rank_predictor
+        self._initialize_zero_state()
+
+    def _initialize_zero_state(self):
+        """adaptive_mps.py:128-139."""
+        for T in self.tensors:
+            T[:, 1, :] = 0
+            T[:, 0, :] = torch.eye(T.shape[0], T.shape[2], dtype=T.dtype, device=T.device)
+        self._normalize()
+
+    # ------------------------------------------------------------ statistics
+    def _log_operation(self, **kwargs):
+        self.statistics.record(step=self._operation_counter, **kwargs)
+        self._operation_counter += 1
+
+    def stats_summary(self) -> Dict[str, float]:
+        return self.statistics.summary()
+
+    def global_error_bound(self) -> float:
+        return self.statistics.global_error_bound()
+
+    def reset_stats(self):
+        self.statistics.reset()
+        self._operation_counter = 0
+
+    # ------------------------------------------------------------ gates
+    @torch.no_grad()
+    def apply_single_qubit_gate(self, q: int, U2: torch.Tensor):
+        """adaptive_mps.py:159-180: T'[a,t,b] = sum_s U[s,t] T[a,s,b]."""
+        assert 0 <= q < self.num_qubits, f"Qubit index {q} out of range"
+        assert tuple(U2.shape) == (2, 2), "U2 must be 2x2"
+        self.tensors[q] = K.apply_1q(self.tensors[q], K.gate_to_host(U2, (2, 2)))
+        self.is_canonical = False
+
+    def _budget_ok_factory(self, cl: int, cm: int, cr: int, local: torch.dtype):
+        def budget_ok(k: int) -> bool:  # adaptive_mps.py:255-264
+            if self.budget_global_mb is None:
+                return True
+            bpe = torch.finfo(local).bits // 8
+            before = (cl * cm + cm * cr) * 2 * bpe
+            after = (cl * k + k * cr) * 2 * bpe
+            delta = max(0, after - before)
+            cur = self.memory_usage() / (1024 ** 2)
+            return (cur + delta / (1024 ** 2)) < self.budget_global_mb
+        return budget_ok
+
+    @torch.no_grad()
+    def apply_two_site_gate(self, i: int, U4: torch.Tensor):
+        """adaptive_mps.py:182-308."""
+        assert 0 <= i < self.num_qubits - 1, f"Bond index {i} out of range"
+        start = time.time()
+        A, B = self.tensors[i], self.tensors[i + 1]
+        cl, cm, cr = A.shape[0], A.shape[2], B.shape[2]
+        local = self.dtype_policy.default
+        A = A.to(dtype=local)
+        B = B.to(dtype=local)
+        gate = _gate4(U4)
+        cap = self.chi_max_per_bond[i]
+        budget_ok = self._budget_ok_factory(cl, cm, cr, local)
+
+        if self.svd_method != "plain":
+            from . import fastsvd
+            res = fastsvd.two_site_update(self, i, A, B, gate, cap, budget_ok, local)
+            A_new, B_new, k, eps_loc, entropy, condS, driver, local = res
+        else:
+            prec = self.theta_precision if local == torch.complex64 else "native"
+            X = K.theta(A, B, gate, prec=prec)
+            U, S, Vh, driver = robust_svd(X)
+            k, eps_loc, entropy, condS = choose_rank_from_sigma(S, self.eps_bond, cap, budget_ok)
+            if math.isfinite(condS) and condS > self.dtype_policy.promote_if_cond_gt:
+                U, S, Vh, driver = robust_svd(X.to(torch.complex128))
+                k, eps_loc, entropy, condS = choose_rank_from_sigma(S, self.eps_bond, cap,
+                                                                    budget_ok)
+                local = torch.complex128
+            A_new, B_new = K.rebuild(U, S, Vh, k, cl, cr)
+        if not check_entropy_sanity(entropy, cl, cr):
+            print(f"Warning: Entropy {entropy:.3f} exceeds physical bound at bond {i}")
+        self.tensors[i] = A_new
+        self.tensors[i + 1] = B_new
+        self.bond_dims[i] = k
+        self._log_operation(bond=i, k_star=k, chi_before=cm, chi_after=k, eps_local=eps_loc,
+                            entropy=entropy, svd_driver=driver, dtype=str(local),
+                            ms_elapsed=(time.time() - start) * 1000, condS=condS)
+        self.is_canonical = False
+
+    # ------------------------------------------------------------ canonical forms
+    @torch.no_grad()
+    def to_left_canonical(self):
+        """adaptive_mps.py:310-339."""
+        for i in range(self.num_qubits - 1):
+            A = self.tensors[i]
+            cl, p, cr = A.shape
+            Q, R, _ = robust_qr(A.reshape(cl * p, cr))
+            cmid = Q.shape[1]
+            self.tensors[i] = Q.reshape(cl, p, cmid)
+            self._absorb_right(i + 1, R)
+            if hasattr(self, "bond_dims") and i < len(self.bond_dims):
+                self.bond_dims[i] = cmid
+        self.is_canonical = True
+
+    def canonicalize_left_to_right(self):
+        """adaptive_mps.py:342-344."""
+        self.to_left_canonical()
+
+    @torch.no_grad()
+    def to_mixed_canonical(self, center: int):
+        """adaptive_mps.py:346-385 (does not touch is_canonical / bond_dims)."""
+        assert 0 <= center < self.num_qubits, "Center out of range"
+        for i in range(center):
+            A = self.tensors[i]
+            cl, p, cr = A.shape
+            Q, R, _ = robust_qr(A.reshape(cl * p, cr))
+            self.tensors[i] = Q.reshape(cl, p, Q.shape[1])
+            self._absorb_right(i + 1, R)
+        for i in range(self.num_qubits - 1, center, -1):
+            B = self.tensors[i]
+            cl, p, cr = B.shape
+            Q, R, _ = robust_qr(B.permute(1, 2, 0).reshape(p * cr, cl))
+            self.tensors[i] = Q.t().reshape(Q.shape[1], p, cr)
+            if i > 0:
+                self._absorb_left(i - 1, R.t())
+
+    # ------------------------------------------------------------ checkpoint
+    def snapshot(self, path: str):
+        """adaptive_mps.py:387-404 (same dict; ``dtype`` added)."""
+        torch.save({"tensors": [t.cpu() for t in self.tensors], "bond_dims": self.bond_dims,
+                    "num_qubits": self.num_qubits, "eps_bond": self.eps_bond,
+                    "chi_max_per_bond": self.chi_max_per_bond,
+                    "statistics": self.statistics.logs, "dtype": str(self.dtype)}, path)
+
+    @staticmethod
+    def load_snapshot(path: str, device: str = "cuda") -> "AdaptiveMPS":
+        """adaptive_mps.py:406-433."""
+        data = torch.load(path, weights_only=True)
+        mps = AdaptiveMPS(num_qubits=data["num_qubits"], bond_dim=max(data["bond_dims"]),
+                          eps_bond=data["eps_bond"], chi_max_per_bond=data["chi_max_per_bond"],
+                          device=device)
+        mps.tensors = [t.to(mps.device) for t in data["tensors"]]
+        mps.bond_dims = data["bond_dims"]
+        if "statistics" in data:
+            mps.statistics.logs = data["statistics"]
+        return mps
+
+    def get_memory_usage(self) -> int:
+        return self.memory_usage()
+
+    @torch.no_grad()
+    def to_statevector(self) -> torch.Tensor:
+        """adaptive_mps.py:444-481 (device tensor of 2^n amplitudes)."""
+        if self.num_qubits > 20:
+            raise ValueError(
+                f"Converting {self.num_qubits} qubits to statevector would require "
+                f"{2 ** self.num_qubits} amplitudes. Use get_amplitude() for large systems.")
+        return self._contract_all(self.dtype)
+
+    # ------------------------------------------------------------ named gates
+    def _g2(self, q: int, m: np.ndarray):
+        self.apply_single_qubit_gate(q, torch.from_numpy(m))
+
+    def h(self, qubit: int):
+        self._g2(qubit, _H)
+
+    def x(self, qubit: int):
+        self._g2(qubit, _X)
+
+    def y(self, qubit: int):
+        self._g2(qubit, _Y)
+
+    def z(self, qubit: int):
+        self._g2(qubit, _Z)
+
+    def s(self, qubit: int):
+        self._g2(qubit, _S)
+
+    def sdg(self, qubit: int):
+        self._g2(qubit, _SDG)
+
+    def t(self, qubit: int):
+        self._g2(qubit, _T)
+
+    def tdg(self, qubit: int):
+        self._g2(qubit, _TDG)
+
+    def rx(self, qubit: int, theta):
+        self._g2(qubit, rx_matrix(theta))
+
+    def ry(self, qubit: int, theta):
+        self._g2(qubit, ry_matrix(theta))
+
+    def rz(self, qubit: int, theta):
+        self._g2(qubit, rz_matrix(theta))
+
+    def cnot(self, control: int, target: int):
+        """adaptive_mps.py:553-571 (adjacent, control < target only)."""
+        if abs(control - target) != 1:
+            raise NotImplementedError("Only adjacent qubit gates supported in MPS")
+        if control > target:
+            raise NotImplementedError(
+                "Non-adjacent CNOT or reversed control/target not yet supported")
+        self.apply_two_site_gate(control, torch.from_numpy(_CNOT))
+
+    def cx(self, control: int, target: int):
+        self.cnot(control, target)
+
+    def cz(self, q0: int, q1: int):
+        if abs(q0 - q1) != 1:
+            raise NotImplementedError("Only adjacent qubit gates supported in MPS")
+        self.apply_two_site_gate(min(q0, q1), torch.from_numpy(_CZ))
+
+    def cy(self, control: int, target: int):
+        if abs(control - target) != 1:
+            raise NotImplementedError("Only adjacent qubit gates supported in MPS")
+        if control > target:
+            raise NotImplementedError("Reversed control/target not yet supported")
+        self.apply_two_site_gate(control, torch.from_numpy(_CY))
+
+    def swap(self, q0: int, q1: int):
+        if abs(q0 - q1) != 1:
+            raise NotImplementedError("Only adjacent qubit gates supported in MPS")
+        self.apply_two_site_gate(min(q0, q1), torch.from_numpy(_SWAP))
+
+    def sample(self, num_shots: int = 1):
+        """adaptive_mps.py:621-623."""
+        return self.measure(num_shots)

@@ -1,0 +1,190 @@
+"""Typed wrappers over the HIP entry points, taking torch tensors on a HIP
+device.  torch provides device memory and the current stream; every
+computation runs in the hand-written kernels of ``libatlasq.so``.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import List, Sequence, Tuple
+
+import numpy as np
+import torch
+
+from . import _ffi
+from ._ffi import RankResult, check, lib
+
+PREC = {"native": _ffi.AQ_PREC_NATIVE, "fp32": _ffi.AQ_PREC_NATIVE,
+        "bf16x3": _ffi.AQ_PREC_BF16X3}
+
+
+def _code(dtype: torch.dtype) -> int:
+    if dtype == torch.complex64:
+        return _ffi.AQ_C64
+    if dtype == torch.complex128:
+        return _ffi.AQ_C128
+    raise TypeError(f"atlas_q kernels take complex64/complex128, got {dtype}")
+
+
+def _real_code(dtype: torch.dtype) -> int:
+    if dtype == torch.float32:
+        return _ffi.AQ_C64
+    if dtype == torch.float64:
+        return _ffi.AQ_C128
+    raise TypeError(f"singular values must be float32/float64, got {dtype}")
+
+
+def _dev(t: torch.Tensor, name: str) -> None:
+    if t.device.type != "cuda":
+        raise RuntimeError(f"atlas_q: {name} must live on a HIP device (got {t.device}); "
+                           "this build has no CPU path")
+
+
+def _stream(t: torch.Tensor) -> ctypes.c_void_p:
+    return ctypes.c_void_p(torch.cuda.current_stream(t.device).cuda_stream)
+
+
+def _ptr(t: torch.Tensor) -> ctypes.c_void_p:
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def gate_to_host(U, shape: Tuple[int, ...]) -> np.ndarray:
+    """Gate values as interleaved float64 (re, im), row-major, on the host."""
+    if isinstance(U, torch.Tensor):
+        arr = U.detach().to("cpu").numpy()
+    else:
+        arr = np.asarray(U)
+    arr = np.ascontiguousarray(arr.astype(np.complex128).reshape(shape))
+    return arr.reshape(-1).view(np.float64).copy()
+
+
+def _dptr(a: np.ndarray) -> ctypes.c_void_p:
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+# ---------------------------------------------------------------- theta
+def theta(A: torch.Tensor, B: torch.Tensor, gate64: np.ndarray, prec: str = "native",
+          out: torch.Tensor = None) -> torch.Tensor:
+    """X[2l+p, q*chiR+j] = sum U[2p+q, 2p1+p2] sum_m A[l,p1,m] B[m,p2,j]."""
+    _dev(A, "A")
+    _dev(B, "B")
+    if A.dtype != B.dtype:
+        raise TypeError("theta: A and B must share a dtype")
+    if A.dim() != 3 or B.dim() != 3 or A.shape[1] != 2 or B.shape[1] != 2 or A.shape[2] != B.shape[0]:
+        raise ValueError(f"theta: bad shapes {tuple(A.shape)} x {tuple(B.shape)}")
+    A = A.contiguous()
+    B = B.contiguous()
+    cl, cm, cr = A.shape[0], A.shape[2], B.shape[2]
+    if out is None:
+        out = torch.empty((2 * cl, 2 * cr), dtype=A.dtype, device=A.device)
+    assert out.is_contiguous() and tuple(out.shape) == (2 * cl, 2 * cr) and out.dtype == A.dtype
+    g = np.ascontiguousarray(gate64, dtype=np.float64)
+    assert g.size == 32
+    check(lib().aq_theta(_code(A.dtype), PREC[prec], _ptr(A), _ptr(B), _dptr(g), _ptr(out),
+                         cl, cm, cr, _stream(A)), "aq_theta")
+    return out
+
+
+# ---------------------------------------------------------------- rank rule
+_RANK_BYTES = ctypes.sizeof(RankResult)
+
+
+def rank_select_async(S: torch.Tensor, eps: float, cap: int) -> torch.Tensor:
+    """Launch the rank rule; returns the device result buffer (uint8[48])."""
+    _dev(S, "S")
+    S = S.contiguous()
+    buf = torch.empty(_RANK_BYTES, dtype=torch.uint8, device=S.device)
+    check(lib().aq_rank_select(_real_code(S.dtype), _ptr(S), S.numel(), float(eps), int(cap),
+                               _ffi.AQ_RANK_REF, _ptr(buf), _stream(S)), "aq_rank_select")
+    return buf
+
+
+def rank_result(buf: torch.Tensor) -> RankResult:
+    host = buf.cpu().numpy()  # the one device->host sync of a gate update
+    return RankResult.from_buffer_copy(host.tobytes())
+
+
+def rank_select(S: torch.Tensor, eps: float, cap: int) -> RankResult:
+    return rank_result(rank_select_async(S, eps, cap))
+
+
+# ---------------------------------------------------------------- rebuild
+def rebuild(U: torch.Tensor, S: torch.Tensor, Vh: torch.Tensor, k: int, chiL: int, chiR: int):
+    """A' = (U[:, :k] * S[:k]).reshape(chiL, 2, k), B' = Vh[:k].reshape(k, 2, chiR)."""
+    _dev(U, "U")
+    dt = U.dtype
+    rows, cols = 2 * chiL, 2 * chiR
+    A = torch.empty((chiL, 2, k), dtype=dt, device=U.device)
+    B = torch.empty((k, 2, chiR), dtype=dt, device=U.device)
+    S = S.contiguous()
+    check(lib().aq_rebuild(_code(dt), _ptr(U), U.stride(0), U.stride(1), _ptr(S), _ptr(Vh),
+                           Vh.stride(0), Vh.stride(1), rows, cols, int(k), _ptr(A), _ptr(B),
+                           _stream(U)), "aq_rebuild")
+    return A, B
+
+
+# ---------------------------------------------------------------- 1q gate
+def apply_1q(T: torch.Tensor, gate64: np.ndarray) -> torch.Tensor:
+    """T'[a,t,b] = sum_s U[s,t] T[a,s,b] (AdaptiveMPS convention)."""
+    _dev(T, "T")
+    T = T.contiguous()
+    out = torch.empty_like(T)
+    g = np.ascontiguousarray(gate64, dtype=np.float64)
+    check(lib().aq_apply_1q(_code(T.dtype), _ptr(T), _ptr(out), T.shape[0], T.shape[2],
+                            _dptr(g), _stream(T)), "aq_apply_1q")
+    return out
+
+
+# ---------------------------------------------------------------- GEMM
+def cgemm(A: torch.Tensor, B: torch.Tensor, conj_a: bool = False, conj_b: bool = False,
+          prec: str = "native", out: torch.Tensor = None) -> torch.Tensor:
+    """C = op(A) @ op(B) for 2-D complex tensors (any strides)."""
+    _dev(A, "A")
+    if A.dtype != B.dtype or A.dim() != 2 or B.dim() != 2 or A.shape[1] != B.shape[0]:
+        raise ValueError(f"cgemm: bad operands {tuple(A.shape)} {A.dtype} x {tuple(B.shape)} {B.dtype}")
+    M, K = A.shape
+    N = B.shape[1]
+    if out is None:
+        out = torch.empty((M, N), dtype=A.dtype, device=A.device)
+    check(lib().aq_cgemm(_code(A.dtype), PREC[prec], M, N, K, _ptr(A), A.stride(0), A.stride(1),
+                         int(conj_a), _ptr(B), B.stride(0), B.stride(1), int(conj_b), _ptr(out),
+                         out.stride(0), out.stride(1), _stream(A)), "aq_cgemm")
+    return out
+
+
+# ---------------------------------------------------------------- sampler
+def sample_bits(tensors: Sequence[torch.Tensor], uniforms: torch.Tensor, prec: str = "native",
+                want_p0: bool = False, single_shot_rules: bool = False):
+    """Run the fused sweep sampler; returns (bits uint64 [S, ceil(n/64)] as int64
+    tensor, p0 float64 [S, n] or None)."""
+    n = len(tensors)
+    dt = tensors[0].dtype
+    dev = tensors[0].device
+    _dev(tensors[0], "site tensors")
+    ts = [t if (t.dtype == dt and t.is_contiguous()) else t.to(dt).contiguous() for t in tensors]
+    chi = [1] + [int(t.shape[2]) for t in ts]
+    for i, t in enumerate(ts):
+        if int(t.shape[0]) != chi[i] or int(t.shape[1]) != 2:
+            raise ValueError(f"sampler: site {i} has shape {tuple(t.shape)}, expected left bond {chi[i]}")
+    if chi[-1] != 1:
+        raise ValueError("sampler: last site must have right bond 1")
+    S = int(uniforms.shape[0])
+    assert uniforms.dtype == torch.float64 and tuple(uniforms.shape) == (S, n)
+    uniforms = uniforms.contiguous()
+    chi_arr = (ctypes.c_int64 * (n + 1))(*chi)
+    ptrs = (ctypes.c_void_p * n)(*[t.data_ptr() for t in ts])
+    ws_bytes = lib().aq_sample_workspace_bytes(_code(dt), n, chi_arr, S)
+    if ws_bytes < 0:
+        raise RuntimeError("aq_sample_workspace_bytes failed")
+    ws = torch.empty(int(ws_bytes), dtype=torch.uint8, device=dev)
+    words = (n + 63) // 64
+    bits = torch.empty((S, words), dtype=torch.int64, device=dev)
+    p0 = torch.empty((S, n), dtype=torch.float64, device=dev) if want_p0 else None
+    check(lib().aq_sample(_code(dt), PREC[prec], int(bool(single_shot_rules)), n, ptrs, chi_arr, S, _ptr(uniforms), _ptr(bits),
+                          _ptr(p0) if p0 is not None else None, _ptr(ws), int(ws_bytes),
+                          _stream(tensors[0])), "aq_sample")
+    return bits, p0
+
+
+def bits_to_ints(bits: torch.Tensor) -> List[int]:
+    """The reference's sample ints: word 0 (low 64 bits) as signed int64."""
+    return bits[:, 0].cpu().tolist()

@@ -1,0 +1,47 @@
+"""SVD / QR with the reference's retry cascade (``src/atlas_q/linalg_robust.py``).
+
+The factorizations run on the HIP device (rocSOLVER through torch).  The
+reference's last resort -- a CPU SVD/QR (linalg_robust.py:49-52, 72-76) -- is
+deliberately absent: a failure after the jitter retry raises.
+"""
+from __future__ import annotations
+
+from typing import Tuple
+
+import torch
+
+
+def _require_device(X: torch.Tensor) -> None:
+    if X.device.type != "cuda":
+        raise RuntimeError("atlas_q: linear algebra runs on the HIP device only "
+                           f"(got a tensor on {X.device})")
+
+
+def robust_svd(X: torch.Tensor) -> Tuple[torch.Tensor, torch.Tensor, torch.Tensor, str]:
+    """linalg_robust.py:17-52 without the CPU step: (U, S, Vh, driver)."""
+    _require_device(X)
+    try:
+        U, S, Vh = torch.linalg.svd(X, full_matrices=False)
+        return U, S, Vh, "torch_cuda"
+    except Exception:
+        pass
+    Xj = X + (1e-12 * torch.randn_like(X))
+    try:
+        U, S, Vh = torch.linalg.svd(Xj, full_matrices=False)
+    except Exception as e:
+        raise RuntimeError(f"atlas_q: device SVD failed after jitter retry: {e}") from e
+    return U, S, Vh, "torch_cuda_jitter"
+
+
+def robust_qr(X: torch.Tensor) -> Tuple[torch.Tensor, torch.Tensor, str]:
+    """linalg_robust.py:55-76 without the CPU step: (Q, R, driver)."""
+    _require_device(X)
+    Q, R = torch.linalg.qr(X)
+    return Q, R, "torch_cuda"
+
+
+def condition_number(S: torch.Tensor) -> float:
+    """linalg_robust.py:79-91."""
+    if len(S) == 0 or S[-1] == 0:
+        return float("inf")
+    return float((S[0] / S[-1]).item())

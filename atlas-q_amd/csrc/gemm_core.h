@@ -1,0 +1,291 @@
+// Complex GEMM mainloop on CDNA4 MFMA (gfx950), shared by the theta
+// contraction, the sweep sampler, the generic C = op(A) op(B) entry point and
+// the projections of the truncated-SVD variants.
+//
+// Work-group tile: 64 rows x 2 "halves" x 64 columns per half, 256 threads =
+// 4 waves in a 2 x 2 arrangement; wave (wm, wn) owns rows [32 wm, 32 wm + 32)
+// and columns [32 wn, 32 wn + 32) of BOTH halves.  For the MPS kernels the two
+// halves are the physical index of the right site (p2 = 0 / 1), so one lane
+// holds, for a given (l, j), all four products Psi[2l+p1, p2*chiR + j]: the
+// 4x4 gate (theta) or the two outcome probabilities (sampler) are applied in
+// registers with no LDS round trip.  For a plain GEMM the halves are just the
+// left and right 64 columns of a 128-column tile.
+//
+// Complex products use the 4-multiplication form on real MFMA:
+//   Cr += Ar Br - Ai Bi,  Ci += Ar Bi + Ai Br
+// FP32: v_mfma_f32_32x32x2f32 (one 32x32 block per wave and half);
+// FP64: v_mfma_f64_16x16x4f64 (2 x 2 blocks of 16x16 per wave and half).
+// Operand tiles are staged through LDS as separate real / imaginary planes,
+// k-major, so each MFMA operand fetch is one conflict-free ds_read per lane.
+#pragma once
+
+#include "aq_common.h"
+
+namespace aq {
+
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+template <typename T> struct Mfma;
+
+template <> struct Mfma<float> {
+  static constexpr int E = 32;   // block edge
+  static constexpr int TK = 2;   // k per instruction
+  static constexpr int NREG = 16;
+  using acc_t = f32x16;
+  static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0);
+  }
+  // operand lane layout: lane holds A[op_idx][op_k] / B[op_k][op_idx]
+  static __device__ __forceinline__ int op_idx(int lane) { return lane & 31; }
+  static __device__ __forceinline__ int op_k(int lane) { return lane >> 5; }
+  // accumulator layout: register r of lane -> (out_row, out_col)
+  static __device__ __forceinline__ int out_row(int lane, int r) {
+    return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+  }
+  static __device__ __forceinline__ int out_col(int lane) { return lane & 31; }
+  static constexpr int COL_LANES = 32;  // lanes sharing a row set
+};
+
+template <> struct Mfma<double> {
+  static constexpr int E = 16;
+  static constexpr int TK = 4;
+  static constexpr int NREG = 4;
+  using acc_t = f64x4;
+  static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int op_idx(int lane) { return lane & 15; }
+  static __device__ __forceinline__ int op_k(int lane) { return lane >> 4; }
+  static __device__ __forceinline__ int out_row(int lane, int r) { return 4 * (lane >> 4) + r; }
+  static __device__ __forceinline__ int out_col(int lane) { return lane & 15; }
+  static constexpr int COL_LANES = 16;
+};
+
+constexpr int kBM = 64;        // rows per work-group tile
+constexpr int kBJ = 64;        // columns per half
+constexpr int kBN = 2 * kBJ;   // columns per tile
+constexpr int kThreads = 256;
+
+template <typename T> struct TileK;
+template <> struct TileK<float> { static constexpr int value = 16; };
+template <> struct TileK<double> { static constexpr int value = 8; };
+
+// ---------------------------------------------------------------- operands
+// A(m, k) = p[m*s0 + k*s1]
+template <typename T> struct AStrided {
+  const c2_t<T>* p;
+  int64_t s0, s1, M, K;
+  int conj;
+  __device__ __forceinline__ c2_t<T> operator()(int64_t m, int64_t k) const {
+    c2_t<T> v;
+    if (m < M && k < K) {
+      v = p[m * s0 + k * s1];
+      if (conj) v.y = -v.y;
+    } else {
+      v.x = 0; v.y = 0;
+    }
+    return v;
+  }
+};
+
+// Sampler left states: A(m, k) = base[m*ld + off[m] + k]; base == nullptr means
+// the all-ones [M, 1] boundary vector.
+template <typename T> struct AGather {
+  const c2_t<T>* base;
+  const int32_t* off;
+  int64_t ld, M, K;
+  __device__ __forceinline__ c2_t<T> operator()(int64_t m, int64_t k) const {
+    c2_t<T> v;
+    v.x = 0; v.y = 0;
+    if (m < M && k < K) {
+      if (base == nullptr) {
+        v.x = 1;
+      } else {
+        v = base[m * ld + (off ? off[m] : 0) + k];
+      }
+    }
+    return v;
+  }
+};
+
+// B(k, n) = p[k*s0 + n*s1]
+template <typename T> struct BStrided {
+  const c2_t<T>* p;
+  int64_t s0, s1, K, N;
+  int conj;
+  __device__ __forceinline__ c2_t<T> operator()(int64_t k, int64_t n) const {
+    c2_t<T> v;
+    if (k < K && n < N) {
+      v = p[k * s0 + n * s1];
+      if (conj) v.y = -v.y;
+    } else {
+      v.x = 0; v.y = 0;
+    }
+    return v;
+  }
+};
+
+// Column map of a tile: PAIRED -> half h, column jj maps to h*half_n + j0 + jj
+// (valid when j0 + jj < half_n); linear -> n0 + h*kBJ + jj (valid when < N).
+struct ColMap {
+  int paired;
+  int64_t half_n;  // paired: columns per half (chiR); linear: N
+  __device__ __forceinline__ int64_t col(int64_t tile_j0, int h, int jj) const {
+    if (paired) {
+      int64_t j = tile_j0 + jj;
+      return j < half_n ? h * half_n + j : INT64_MAX;
+    }
+    int64_t n = 2 * tile_j0 + h * kBJ + jj;
+    return n < half_n ? n : INT64_MAX;
+  }
+};
+
+// ---------------------------------------------------------------- mainloop
+// Accumulator fragment of one wave: [RB][JB][2 halves] blocks of E x E.
+template <typename T> struct Frag {
+  static constexpr int E = Mfma<T>::E;
+  static constexpr int RB = 32 / E;
+  static constexpr int JB = 32 / E;
+  typename Mfma<T>::acc_t re[RB][JB][2];
+  typename Mfma<T>::acc_t im[RB][JB][2];
+};
+
+template <typename T> struct alignas(16) Smem {
+  static constexpr int BK = TileK<T>::value;
+  static constexpr int PAD = 4;
+  T a[2][BK][kBM + PAD];
+  T b[2][BK][kBN + PAD];
+};
+
+// k_fast_a: the A functor is contiguous along k (else along m).
+// n_fast_b: the B functor is contiguous along n (else along k).
+template <typename T, bool KFastA, bool NFastB, class AOp, class BOp>
+__device__ __forceinline__ void cgemm_mainloop(const AOp& aop, const BOp& bop, const ColMap& cm,
+                                               int64_t K, int64_t m0, int64_t j0, Smem<T>& sm,
+                                               Frag<T>& fr) {
+  using M = Mfma<T>;
+  constexpr int BK = Smem<T>::BK;
+  constexpr int E = M::E;
+  constexpr int A_PER = kBM * BK / kThreads;
+  constexpr int B_PER = kBN * BK / kThreads;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+
+#pragma unroll
+  for (int rb = 0; rb < Frag<T>::RB; ++rb)
+#pragma unroll
+    for (int jb = 0; jb < Frag<T>::JB; ++jb)
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int r = 0; r < M::NREG; ++r) {
+          fr.re[rb][jb][h][r] = 0;
+          fr.im[rb][jb][h][r] = 0;
+        }
+
+  // global column of each B element this thread loads is loop-invariant
+  int64_t bcol[B_PER];
+  int bkk[B_PER], btc[B_PER];
+#pragma unroll
+  for (int i = 0; i < B_PER; ++i) {
+    int e = tid + i * kThreads;
+    int tc, kk;
+    if (NFastB) { tc = e % kBN; kk = e / kBN; }
+    else        { kk = e % BK;  tc = e / BK; }
+    bkk[i] = kk;
+    btc[i] = tc;
+    bcol[i] = cm.col(j0, tc / kBJ, tc % kBJ);
+  }
+  int amm[A_PER], akk[A_PER];
+#pragma unroll
+  for (int i = 0; i < A_PER; ++i) {
+    int e = tid + i * kThreads;
+    if (KFastA) { akk[i] = e % BK; amm[i] = e / BK; }
+    else        { amm[i] = e % kBM; akk[i] = e / kBM; }
+  }
+
+  c2_t<T> ra[A_PER], rbv[B_PER];
+  auto fetch = [&](int64_t k0) {
+#pragma unroll
+    for (int i = 0; i < A_PER; ++i) ra[i] = aop(m0 + amm[i], k0 + akk[i]);
+#pragma unroll
+    for (int i = 0; i < B_PER; ++i) {
+      if (bcol[i] != INT64_MAX) {
+        rbv[i] = bop(k0 + bkk[i], bcol[i]);
+      } else {
+        rbv[i].x = 0; rbv[i].y = 0;
+      }
+    }
+  };
+
+  fetch(0);
+  for (int64_t k0 = 0; k0 < K; k0 += BK) {
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < A_PER; ++i) {
+      sm.a[0][akk[i]][amm[i]] = ra[i].x;
+      sm.a[1][akk[i]][amm[i]] = ra[i].y;
+    }
+#pragma unroll
+    for (int i = 0; i < B_PER; ++i) {
+      sm.b[0][bkk[i]][btc[i]] = rbv[i].x;
+      sm.b[1][bkk[i]][btc[i]] = rbv[i].y;
+    }
+    __syncthreads();
+    if (k0 + BK < K) fetch(k0 + BK);  // overlap next tile's loads with MFMA
+
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += M::TK) {
+      const int kr = kk + M::op_k(lane);
+      T ar[Frag<T>::RB], ai[Frag<T>::RB], nai[Frag<T>::RB];
+#pragma unroll
+      for (int rb = 0; rb < Frag<T>::RB; ++rb) {
+        const int row = 32 * wm + rb * E + M::op_idx(lane);
+        ar[rb] = sm.a[0][kr][row];
+        ai[rb] = sm.a[1][kr][row];
+        nai[rb] = -ai[rb];
+      }
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int jb = 0; jb < Frag<T>::JB; ++jb) {
+          const int col = h * kBJ + 32 * wn + jb * E + M::op_idx(lane);
+          const T br = sm.b[0][kr][col];
+          const T bi = sm.b[1][kr][col];
+#pragma unroll
+          for (int rb = 0; rb < Frag<T>::RB; ++rb) {
+            fr.re[rb][jb][h] = M::mma(ar[rb], br, fr.re[rb][jb][h]);
+            fr.re[rb][jb][h] = M::mma(nai[rb], bi, fr.re[rb][jb][h]);
+            fr.im[rb][jb][h] = M::mma(ar[rb], bi, fr.im[rb][jb][h]);
+            fr.im[rb][jb][h] = M::mma(ai[rb], br, fr.im[rb][jb][h]);
+          }
+        }
+    }
+  }
+}
+
+// Visit every accumulator element of this thread: f(tile_row, h, tile_jj, re, im)
+// where tile_row in [0, 64), tile_jj in [0, 64).
+template <typename T, class F>
+__device__ __forceinline__ void frag_foreach(const Frag<T>& fr, F&& f) {
+  using M = Mfma<T>;
+  constexpr int E = M::E;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+#pragma unroll
+  for (int rb = 0; rb < Frag<T>::RB; ++rb)
+#pragma unroll
+    for (int jb = 0; jb < Frag<T>::JB; ++jb)
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int r = 0; r < M::NREG; ++r)
+          f(32 * wm + rb * E + M::out_row(lane, r), h, 32 * wn + jb * E + M::out_col(lane),
+            fr.re[rb][jb][h][r], fr.im[rb][jb][h][r]);
+}
+
+}  // namespace aq

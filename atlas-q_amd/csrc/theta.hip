@@ -1,0 +1,232 @@
+// Theta contraction with the two-qubit gate fused into the epilogue, and the
+// general complex GEMM entry point.
+//
+//   Psi[2l+p1, p2*chiR + j] = sum_m A[l,p1,m] B[m,p2,j]     (MFMA mainloop)
+//   X[2l+p, q*chiR + j]     = sum_{p1,p2} U[2p+q, 2p1+p2] Psi[2l+p1, p2*chiR+j]
+//
+// The work-group tile pairs rows (2l, 2l+1) in accumulator registers (r, r+1)
+// and the two column halves (p2 = 0, 1) in the two half-blocks, so the 4x4
+// gate is applied per lane in registers and X is written once, coalesced
+// along j.  Replaces fused_two_qubit_gate_pytorch
+// (src/triton_kernels/mps_complex.py:343-368).
+#include "gemm_core.h"
+#include "bf16x3.h"
+
+namespace aq {
+
+template <typename T> struct Gate4 {
+  T re[16];
+  T im[16];
+};
+
+template <typename T, class FR>
+__device__ __forceinline__ void theta_epilogue(const FR& fr, const Gate4<T>& g, c2_t<T>* X,
+                                               int64_t chiL, int64_t chiR, int64_t m0,
+                                               int64_t j0) {
+  using M = Mfma<T>;
+  constexpr int E = M::E;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int64_t ld = 2 * chiR;
+#pragma unroll
+  for (int rb = 0; rb < FR::RB; ++rb)
+#pragma unroll
+    for (int jb = 0; jb < FR::JB; ++jb)
+#pragma unroll
+      for (int r = 0; r < M::NREG; r += 2) {
+        const int64_t row = m0 + 32 * wm + rb * E + M::out_row(lane, r);  // even: 2l
+        const int64_t j = j0 + 32 * wn + jb * E + M::out_col(lane);
+        if (row >= 2 * chiL || j >= chiR) continue;
+        T pr[4], pi[4];  // index 2*p1 + p2
+#pragma unroll
+        for (int p1 = 0; p1 < 2; ++p1)
+#pragma unroll
+          for (int p2 = 0; p2 < 2; ++p2) {
+            pr[2 * p1 + p2] = fr.re[rb][jb][p2][r + p1];
+            pi[2 * p1 + p2] = fr.im[rb][jb][p2][r + p1];
+          }
+#pragma unroll
+        for (int p = 0; p < 2; ++p)
+#pragma unroll
+          for (int q = 0; q < 2; ++q) {
+            const int o = 2 * p + q;
+            T xr = 0, xi = 0;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+              xr += g.re[4 * o + c] * pr[c] - g.im[4 * o + c] * pi[c];
+              xi += g.re[4 * o + c] * pi[c] + g.im[4 * o + c] * pr[c];
+            }
+            c2_t<T> v;
+            v.x = xr;
+            v.y = xi;
+            X[(row + p) * ld + q * chiR + j] = v;
+          }
+      }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void theta_kernel(const c2_t<T>* __restrict__ A,
+                                                         const c2_t<T>* __restrict__ B,
+                                                         Gate4<T> g, c2_t<T>* __restrict__ X,
+                                                         int64_t chiL, int64_t chiM,
+                                                         int64_t chiR) {
+  __shared__ Smem<T> sm;
+  const int64_t m0 = (int64_t)blockIdx.x * kBM;
+  const int64_t j0 = (int64_t)blockIdx.y * kBJ;
+  AStrided<T> aop{A, chiM, 1, 2 * chiL, chiM, 0};
+  BStrided<T> bop{B, 2 * chiR, 1, chiM, 2 * chiR, 0};
+  ColMap cm{1, chiR};
+  Frag<T> fr;
+  cgemm_mainloop<T, true, true>(aop, bop, cm, chiM, m0, j0, sm, fr);
+  theta_epilogue<T>(fr, g, X, chiL, chiR, m0, j0);
+}
+
+__global__ __launch_bounds__(kThreads) void theta_bf16x3_kernel(
+    const float2* __restrict__ A, const float2* __restrict__ B, Gate4<float> g,
+    float2* __restrict__ X, int64_t chiL, int64_t chiM, int64_t chiR) {
+  __shared__ SmemBf16 sm;
+  const int64_t m0 = (int64_t)blockIdx.x * kBM;
+  const int64_t j0 = (int64_t)blockIdx.y * kBJ;
+  AStrided<float> aop{A, chiM, 1, 2 * chiL, chiM, 0};
+  BStrided<float> bop{B, 2 * chiR, 1, chiM, 2 * chiR, 0};
+  ColMap cm{1, chiR};
+  Frag<float> fr;
+  cgemm_mainloop_bf16x3<true, true>(aop, bop, cm, chiM, m0, j0, sm, fr);
+  theta_epilogue<float>(fr, g, X, chiL, chiR, m0, j0);
+}
+
+template <typename T, bool KFastA, bool NFastB>
+__global__ __launch_bounds__(kThreads) void cgemm_kernel(AStrided<T> aop, BStrided<T> bop,
+                                                         c2_t<T>* __restrict__ C, int64_t M,
+                                                         int64_t N, int64_t K, int64_t c_s0,
+                                                         int64_t c_s1) {
+  __shared__ Smem<T> sm;
+  const int64_t m0 = (int64_t)blockIdx.x * kBM;
+  const int64_t j0 = (int64_t)blockIdx.y * kBJ;
+  ColMap cm{0, N};
+  Frag<T> fr;
+  cgemm_mainloop<T, KFastA, NFastB>(aop, bop, cm, K, m0, j0, sm, fr);
+  frag_foreach<T>(fr, [&](int row, int h, int jj, T re, T im) {
+    const int64_t m = m0 + row;
+    const int64_t n = 2 * j0 + h * kBJ + jj;
+    if (m < M && n < N) {
+      c2_t<T> v;
+      v.x = re;
+      v.y = im;
+      C[m * c_s0 + n * c_s1] = v;
+    }
+  });
+}
+
+template <bool KFastA, bool NFastB>
+__global__ __launch_bounds__(kThreads) void cgemm_bf16x3_kernel(AStrided<float> aop,
+                                                                BStrided<float> bop,
+                                                                float2* __restrict__ C,
+                                                                int64_t M, int64_t N, int64_t K,
+                                                                int64_t c_s0, int64_t c_s1) {
+  __shared__ SmemBf16 sm;
+  const int64_t m0 = (int64_t)blockIdx.x * kBM;
+  const int64_t j0 = (int64_t)blockIdx.y * kBJ;
+  ColMap cm{0, N};
+  Frag<float> fr;
+  cgemm_mainloop_bf16x3<KFastA, NFastB>(aop, bop, cm, K, m0, j0, sm, fr);
+  frag_foreach<float>(fr, [&](int row, int h, int jj, float re, float im) {
+    const int64_t m = m0 + row;
+    const int64_t n = 2 * j0 + h * kBJ + jj;
+    if (m < M && n < N) C[m * c_s0 + n * c_s1] = make_float2(re, im);
+  });
+}
+
+template <typename T>
+static Gate4<T> make_gate(const double* g) {
+  Gate4<T> out;
+  for (int i = 0; i < 16; ++i) {
+    out.re[i] = (T)g[2 * i];
+    out.im[i] = (T)g[2 * i + 1];
+  }
+  return out;
+}
+
+template <typename T, bool KA, bool NB>
+static void launch_cgemm(dim3 grid, hipStream_t s, AStrided<T> a, BStrided<T> b, void* C,
+                         int64_t M, int64_t N, int64_t K, int64_t cs0, int64_t cs1, int prec) {
+  if constexpr (std::is_same<T, float>::value) {
+    if (prec == AQ_PREC_BF16X3) {
+      cgemm_bf16x3_kernel<KA, NB><<<grid, kThreads, 0, s>>>(a, b, (float2*)C, M, N, K, cs0, cs1);
+      return;
+    }
+  }
+  cgemm_kernel<T, KA, NB><<<grid, kThreads, 0, s>>>(a, b, (c2_t<T>*)C, M, N, K, cs0, cs1);
+}
+
+template <typename T>
+static int cgemm_impl(int prec, int64_t M, int64_t N, int64_t K, const void* A, int64_t as0,
+                      int64_t as1, int ca, const void* B, int64_t bs0, int64_t bs1, int cb,
+                      void* C, int64_t cs0, int64_t cs1, hipStream_t s) {
+  AStrided<T> a{(const c2_t<T>*)A, as0, as1, M, K, ca};
+  BStrided<T> b{(const c2_t<T>*)B, bs0, bs1, K, N, cb};
+  dim3 grid((unsigned)cdiv(M, kBM), (unsigned)cdiv(N, kBN));
+  const bool ka = (as1 == 1), nb = (bs1 == 1);
+  if (ka && nb) launch_cgemm<T, true, true>(grid, s, a, b, C, M, N, K, cs0, cs1, prec);
+  else if (ka) launch_cgemm<T, true, false>(grid, s, a, b, C, M, N, K, cs0, cs1, prec);
+  else if (nb) launch_cgemm<T, false, true>(grid, s, a, b, C, M, N, K, cs0, cs1, prec);
+  else launch_cgemm<T, false, false>(grid, s, a, b, C, M, N, K, cs0, cs1, prec);
+  AQ_LAUNCH_CHECK();
+  return AQ_OK;
+}
+
+}  // namespace aq
+
+using namespace aq;
+
+extern "C" int aq_theta(int dtype, int prec, const void* A, const void* B,
+                        const double* gate_host, void* X, int64_t chiL, int64_t chiM,
+                        int64_t chiR, hipStream_t stream) {
+  AQ_REQUIRE(A && B && X && gate_host, AQ_ERR_ARG, "aq_theta: null pointer");
+  AQ_REQUIRE(chiL > 0 && chiM > 0 && chiR > 0, AQ_ERR_ARG,
+             "aq_theta: bond sizes must be positive (got %lld, %lld, %lld)", (long long)chiL,
+             (long long)chiM, (long long)chiR);
+  AQ_REQUIRE(chiL <= (1 << 20) && chiR <= (1 << 20) && chiM <= (1 << 20), AQ_ERR_ARG,
+             "aq_theta: bond size too large");
+  dim3 grid((unsigned)cdiv(2 * chiL, kBM), (unsigned)cdiv(chiR, kBJ));
+  if (dtype == AQ_C64) {
+    if (prec == AQ_PREC_BF16X3) {
+      theta_bf16x3_kernel<<<grid, kThreads, 0, stream>>>((const float2*)A, (const float2*)B,
+                                                         make_gate<float>(gate_host),
+                                                         (float2*)X, chiL, chiM, chiR);
+    } else {
+      AQ_REQUIRE(prec == AQ_PREC_NATIVE, AQ_ERR_ARG, "aq_theta: unknown precision %d", prec);
+      theta_kernel<float><<<grid, kThreads, 0, stream>>>((const float2*)A, (const float2*)B,
+                                                         make_gate<float>(gate_host),
+                                                         (float2*)X, chiL, chiM, chiR);
+    }
+  } else if (dtype == AQ_C128) {
+    AQ_REQUIRE(prec == AQ_PREC_NATIVE, AQ_ERR_ARG,
+               "aq_theta: complex128 supports only native FP64 arithmetic");
+    theta_kernel<double><<<grid, kThreads, 0, stream>>>((const double2*)A, (const double2*)B,
+                                                        make_gate<double>(gate_host),
+                                                        (double2*)X, chiL, chiM, chiR);
+  } else {
+    AQ_REQUIRE(false, AQ_ERR_ARG, "aq_theta: unknown dtype %d", dtype);
+  }
+  AQ_LAUNCH_CHECK();
+  return AQ_OK;
+}
+
+extern "C" int aq_cgemm(int dtype, int prec, int64_t M, int64_t N, int64_t K, const void* A,
+                        int64_t a_s0, int64_t a_s1, int conj_a, const void* B, int64_t b_s0,
+                        int64_t b_s1, int conj_b, void* C, int64_t c_s0, int64_t c_s1,
+                        hipStream_t stream) {
+  AQ_REQUIRE(A && B && C, AQ_ERR_ARG, "aq_cgemm: null pointer");
+  AQ_REQUIRE(M > 0 && N > 0 && K > 0, AQ_ERR_ARG, "aq_cgemm: sizes must be positive");
+  AQ_REQUIRE(cdiv(M, kBM) < (1LL << 31) && cdiv(N, kBN) < 65536, AQ_ERR_ARG,
+             "aq_cgemm: grid too large");
+  if (dtype == AQ_C64)
+    return cgemm_impl<float>(prec, M, N, K, A, a_s0, a_s1, conj_a, B, b_s0, b_s1, conj_b, C,
+                             c_s0, c_s1, stream);
+  AQ_REQUIRE(dtype == AQ_C128, AQ_ERR_ARG, "aq_cgemm: unknown dtype %d", dtype);
+  AQ_REQUIRE(prec == AQ_PREC_NATIVE, AQ_ERR_ARG, "aq_cgemm: complex128 is FP64 only");
+  return cgemm_impl<double>(prec, M, N, K, A, a_s0, a_s1, conj_a, B, b_s0, b_s1, conj_b, C,
+                            c_s0, c_s1, stream);
+}

@@ -1,0 +1,130 @@
+/* atlasq.h -- C ABI of the MI355X (gfx950) adaptive-MPS hot path.
+ *
+ * The reference (ATLAS-Q) has no native ABI on this path: it is Python over
+ * torch.  Each entry point below replaces one reference function or inner
+ * step; the citation names it (paths relative to the reference checkout).
+ *
+ * Conventions
+ *   - Complex data is interleaved (re, im): complex64 = float2, complex128 =
+ *     double2, the torch layout.  Sizes are int64_t element counts.
+ *   - The caller allocates every input, output and workspace (torch owns
+ *     device memory); the library never frees or keeps caller pointers.
+ *   - Every call is asynchronous on `stream`; no call synchronises the
+ *     device.  Returns AQ_OK (0) or a negative AQ_ERR_* status; the message
+ *     is in aq_last_error() (thread-local).
+ *   - Site tensors are row-major [chiL, 2, chiR].  Theta X is row-major
+ *     [2*chiL, 2*chiR] with X[2l+p, q*chiR+j].
+ */
+#ifndef ATLASQ_H
+#define ATLASQ_H
+
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AQ_OK 0
+#define AQ_ERR_ARG (-1)
+#define AQ_ERR_HIP (-2)
+#define AQ_ERR_UNSUPPORTED (-3)
+
+/* arithmetic of the contractions */
+#define AQ_PREC_NATIVE 0 /* FP32 MFMA for complex64, FP64 MFMA for complex128 */
+#define AQ_PREC_BF16X3 1 /* complex64 only: hi/lo BF16 split, 3 MFMAs, FP32 acc */
+
+/* dtype codes */
+#define AQ_C64 0
+#define AQ_C128 1
+
+/* rank-rule arithmetic */
+#define AQ_RANK_REF 0 /* reference CPU semantics in S's dtype (see aq_rank_select) */
+
+int aq_version(void);
+const char* aq_last_error(void);
+
+/* Theta: X = gate-applied two-site contraction, "U" convention
+ *   X[2l+p, q*chiR+j] = sum_{p1,p2} U[2p+q, 2p1+p2] sum_m A[l,p1,m] B[m,p2,j].
+ * Replaces fused_two_qubit_gate_pytorch (src/triton_kernels/mps_complex.py:343-368)
+ * and fused_two_qubit_gate_triton (mps_complex.py:221-336), as called from
+ * AdaptiveMPS.apply_two_site_gate (src/atlas_q/adaptive_mps.py:225-247).
+ * The gate is passed by value from the host: gate_host = 32 doubles,
+ * U[r][c] = (gate_host[2*(4r+c)], gate_host[2*(4r+c)+1]).
+ * A [chiL,2,chiM], B [chiM,2,chiR], X [2chiL,2chiR], all contiguous, dtype by
+ * `dtype` (AQ_C64 / AQ_C128).  prec: AQ_PREC_NATIVE or AQ_PREC_BF16X3 (c64). */
+int aq_theta(int dtype, int prec, const void* A, const void* B, const double* gate_host,
+             void* X, int64_t chiL, int64_t chiM, int64_t chiR, hipStream_t stream);
+
+/* Rank rule: replaces choose_rank_from_sigma (src/atlas_q/truncation.py:21-83)
+ * as called at adaptive_mps.py:266/273.  S: m singular values (float32 when
+ * s_dtype == AQ_C64, float64 when AQ_C128), descending.  cap: per-bond cap
+ * already reduced by the memory budget (adaptive_mps.py:255-264 is evaluated on
+ * the host, where the tensor sizes live).  Writes one aq_rank_result_t to
+ * device memory `out`.  AQ_RANK_REF follows the reference's CPU rounding:
+ * E = cumsum(S*S) accumulated sequentially in double and rounded to S's dtype,
+ * thresh = (1-eps^2 rounded to S's dtype) * total in S's dtype, left
+ * searchsorted, k = min(idx+1, m, cap). */
+typedef struct {
+  int32_t k;
+  int32_t flags; /* bit0: total < 1e-30 (degenerate) */
+  double eps_local;
+  double entropy; /* nats, over the kept values */
+  double condS;   /* S[0]/S[k-1], +inf when k <= 1 or S[k-1] == 0 */
+  double total_weight;
+  double kept_weight;
+} aq_rank_result_t;
+int aq_rank_select(int s_dtype, const void* S, int64_t m, double eps, int64_t cap, int mode,
+                   aq_rank_result_t* out, hipStream_t stream);
+
+/* Core rebuild (adaptive_mps.py:280-290): A' = (U[:, :k] * S[:k]) as
+ * [rows/2, 2, k], B' = Vh[:k, :] as [k, 2, cols/2].  U is rows x >=k with
+ * element (r,c) at U[r*u_s0 + c*u_s1]; Vh is >=k x cols with (r,c) at
+ * Vh[r*v_s0 + c*v_s1] (element strides; torch's SVD factors may be
+ * column-major).  S real (float / double per dtype). */
+int aq_rebuild(int dtype, const void* U, int64_t u_s0, int64_t u_s1, const void* S,
+               const void* Vh, int64_t v_s0, int64_t v_s1, int64_t rows, int64_t cols,
+               int64_t k, void* A_out, void* B_out, hipStream_t stream);
+
+/* Single-qubit gate, AdaptiveMPS convention (adaptive_mps.py:159-180):
+ * T_out[a,t,b] = sum_s U[s,t] T_in[a,s,b].  gate_host: 8 doubles (U[r][c]
+ * interleaved re/im, row-major).  T_in, T_out contiguous [chiL,2,chiR]. */
+int aq_apply_1q(int dtype, const void* T_in, void* T_out, int64_t chiL, int64_t chiR,
+                const double* gate_host, hipStream_t stream);
+
+/* General complex GEMM  C = op(A) op(B)  on MFMA, element strides.
+ * A: M x K, element (m,k) at A[m*a_s0 + k*a_s1], conjugated when conj_a.
+ * B: K x N, element (k,n) at B[k*b_s0 + n*b_s1], conjugated when conj_b.
+ * C: M x N, element (m,n) at C[m*c_s0 + n*c_s1] (overwritten).
+ * Used for the canonical-form absorptions (adaptive_mps.py:333,370,385), the
+ * state-vector contraction (adaptive_mps.py:463-481) and the projections of
+ * the truncated-SVD variants. */
+int aq_cgemm(int dtype, int prec, int64_t M, int64_t N, int64_t K,
+             const void* A, int64_t a_s0, int64_t a_s1, int conj_a,
+             const void* B, int64_t b_s0, int64_t b_s1, int conj_b,
+             void* C, int64_t c_s0, int64_t c_s1, hipStream_t stream);
+
+/* Batched sweep sampler: replaces MatrixProductStatePyTorch._batch_sweep_sample
+ * (src/atlas_q/mps_pytorch.py:247-355) with its multinomial draw expressed as
+ * the inverse CDF  outcome = (u >= p0)  over caller-supplied uniforms.
+ * sites: host array of n device pointers, site i is [chi[i], 2, chi[i+1]]
+ * (chi: host array of n+1 bond sizes, chi[0] = chi[n] = 1), all of `dtype`.
+ * uniforms: device float64 [shots, n].  bits: device uint64
+ * [shots, ceil(n/64)]: qubit i is bit (n-1-i) of the little-endian word
+ * array (qubit 0 = most significant, as mps_pytorch.py:321; word 0 is the
+ * reference's int64 value modulo 2^64).  p0_out (nullable): device float64
+ * [shots, n], the normalised probability of outcome 0 used at each draw.
+ * workspace: device memory of aq_sample_workspace_bytes(...) bytes.
+ * flags: AQ_SAMPLE_SINGLE_SHOT_RULES selects the reference's one-shot path
+ * (mps_pytorch.py:177-245): first-site probabilities |sum_b A[0,s,b]|^2 and
+ * Python-float normalisation without the 1e-15 floor (0.5/0.5 below 1e-15). */
+#define AQ_SAMPLE_SINGLE_SHOT_RULES 1
+int64_t aq_sample_workspace_bytes(int dtype, int64_t n, const int64_t* chi, int64_t shots);
+int aq_sample(int dtype, int prec, int flags, int64_t n, const void* const* sites, const int64_t* chi,
+              int64_t shots, const double* uniforms, uint64_t* bits, double* p0_out,
+              void* workspace, int64_t workspace_bytes, hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ATLASQ_H */

@@ -1,0 +1,94 @@
+"""CPU checks of the drop-in boundary: the C-ABI library loads and exports
+every entry point the header declares, the Python mirror exposes the
+reference's names, host-side logic behaves, and nothing falls back to CPU."""
+import ctypes
+import inspect
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import atlas_q
+from atlas_q import _ffi
+from atlas_q import circuits as C
+from atlas_q.diagnostics import MPSStatistics, bond_entropy_from_S, effective_rank
+from atlas_q.truncation import check_entropy_sanity, compute_global_error_bound
+
+
+def test_library_exports_every_header_symbol():
+    syms = _ffi.header_symbols()
+    assert {"aq_theta", "aq_rank_select", "aq_rebuild", "aq_sample", "aq_cgemm"} <= set(syms)
+    L = _ffi.lib()
+    for s in syms:
+        assert hasattr(L, s), s
+        assert s in _ffi.SIGNATURES, f"{s} has no ctypes signature"
+    assert L.aq_version() >= 10000
+
+
+def test_error_path_reports_message():
+    L = _ffi.lib()
+    rc = L.aq_theta(0, 0, None, None, None, None, 1, 1, 1, None)
+    assert rc == -1
+    assert b"null pointer" in L.aq_last_error()
+    rc = L.aq_sample(0, 0, 0, 0, None, None, 0, None, None, None, None, 0, None)
+    assert rc == -1
+
+
+def test_rank_result_struct_layout():
+    assert ctypes.sizeof(_ffi.RankResult) == 48
+
+
+def test_export_map_matches_reference_names():
+    m = atlas_q.get_adaptive_mps()
+    assert set(m) == {"AdaptiveMPS", "DTypePolicy", "robust_svd", "robust_qr",
+                      "condition_number", "choose_rank_from_sigma",
+                      "compute_global_error_bound", "check_entropy_sanity", "MPSStatistics",
+                      "bond_entropy_from_S", "effective_rank", "spectral_gap"}
+    sig = inspect.signature(m["AdaptiveMPS"].__init__)
+    for name, default in [("bond_dim", 8), ("eps_bond", 1e-6), ("chi_max_per_bond", 256),
+                          ("budget_global_mb", None), ("device", "cuda"), ("dtype", None)]:
+        assert sig.parameters[name].default == default
+    assert m["DTypePolicy"]().default == torch.complex64
+    assert m["DTypePolicy"]().promote_if_cond_gt == 1e6
+
+
+def test_no_cpu_path():
+    from atlas_q.adaptive_mps import AdaptiveMPS
+    from atlas_q import kernels as K
+    with pytest.raises(RuntimeError, match="HIP device"):
+        AdaptiveMPS(4, device="cpu")
+    with pytest.raises(RuntimeError, match="HIP device"):
+        K.theta(torch.zeros(1, 2, 1, dtype=torch.complex64),
+                torch.zeros(1, 2, 1, dtype=torch.complex64), np.zeros(32))
+
+
+def test_statistics_and_bounds_host_logic():
+    st = MPSStatistics()
+    st.record(step=0, bond=0, k_star=4, chi_before=8, chi_after=4, eps_local=1e-6, entropy=1.5,
+              svd_driver="torch_cuda", dtype="complex64", ms_elapsed=10.0, condS=100.0)
+    st.record(step=1, bond=1, k_star=6, chi_before=8, chi_after=6, eps_local=2e-6, entropy=2.0,
+              svd_driver="torch_cuda", dtype="complex64", ms_elapsed=12.0, condS=150.0)
+    s = st.summary()
+    assert s["total_operations"] == 2 and s["max_chi"] == 6 and s["total_time_ms"] == 22.0
+    assert s["cuda_svd_pct"] == 100.0
+    assert abs(st.global_error_bound() - math.sqrt(5e-12)) < 1e-18
+    assert compute_global_error_bound([3.0, 4.0]) == 5.0
+    assert check_entropy_sanity(1.0, 1, 1) and not check_entropy_sanity(1.1, 1, 1)
+    assert abs(bond_entropy_from_S(torch.ones(4) / 2) - 2.0) < 1e-6
+    assert effective_rank(torch.tensor([1.0, 0.1, 0.01, 0.001]), 0.99) <= 2
+
+
+def test_circuits_are_seeded_and_haar_su4():
+    a = C.brickwork(8, 4, 3)
+    b = C.brickwork(8, 4, 3)
+    assert len(a) == C.brickwork_count(8, 4) == 14
+    for (i, U), (j, V) in zip(a, b):
+        assert i == j and np.array_equal(U, V)
+        assert np.allclose(U.conj().T @ U, np.eye(4), atol=1e-12)
+        assert abs(np.linalg.det(U) - 1) < 1e-12
+    assert [op[0] for op in a[:4]] == [0, 2, 4, 6]
+    ops = C.hea(5, 2, 0)
+    assert sum(op[0] == "2q" for op in ops) == 2 * 4
+    pop = C.batched_hea_population(256, 0)
+    assert len(pop) == 256 and min(p[0] for p in pop) >= 12 and max(p[0] for p in pop) <= 96
