@@ -46,6 +46,16 @@ SIGNATURES = {
     "aq_sample_workspace_bytes": (c_i64, [c_int, c_i64, c_vp, c_i64]),
     "aq_sample": (c_int, [c_int, c_int, c_int, c_i64, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp,
                           c_i64, c_vp]),
+    "aq_heev_small": (c_int, [c_i64, c_vp, c_i64, c_i64, c_vp, c_vp, c_i64, c_i64, c_i64, c_int,
+                              ctypes.c_double, c_vp, c_vp]),
+    "aq_mlp_num_params": (c_int, []),
+    "aq_mlp_sync_bf16": (c_int, [c_vp, c_vp, c_vp]),
+    "aq_mlp_forward": (c_int, [c_vp, c_i64, c_vp, c_vp, c_vp, c_vp]),
+    "aq_mlp_workspace_floats": (c_i64, [c_i64]),
+    "aq_mlp_backward": (c_int, [c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp]),
+    "aq_mlp_adamw": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, ctypes.c_double, ctypes.c_double,
+                             ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                             c_int, c_vp]),
 }
 
 _lib = None

@@ -78,6 +78,9 @@ class AdaptiveMPS(MatrixProductStatePyTorch):
         self.svd_method = svd_method
         self.theta_precision = theta_precision
         self.rank_predictor = rank_predictor
+        self.harvest = None          # list -> (features, kept rank) pairs are appended
+        self._last_entropy = {}
+        self._last_eps = {}
         self._initialize_zero_state()
 
     def _initialize_zero_state(self):
@@ -144,14 +147,21 @@ class AdaptiveMPS(MatrixProductStatePyTorch):
         else:
             prec = self.theta_precision if local == torch.complex64 else "native"
             X = K.theta(A, B, gate, prec=prec)
+            feats = None
+            if self.harvest is not None and self.rank_predictor is not None:
+                feats = self.rank_predictor.features(self, i, X, gate, cap)
             U, S, Vh, driver = robust_svd(X)
             k, eps_loc, entropy, condS = choose_rank_from_sigma(S, self.eps_bond, cap, budget_ok)
+            if feats is not None:
+                self.harvest.append((feats.cpu().numpy(), k))
             if math.isfinite(condS) and condS > self.dtype_policy.promote_if_cond_gt:
                 U, S, Vh, driver = robust_svd(X.to(torch.complex128))
                 k, eps_loc, entropy, condS = choose_rank_from_sigma(S, self.eps_bond, cap,
                                                                     budget_ok)
                 local = torch.complex128
             A_new, B_new = K.rebuild(U, S, Vh, k, cl, cr)
+        self._last_entropy[i] = entropy
+        self._last_eps[i] = eps_loc
         if not check_entropy_sanity(entropy, cl, cr):
             print(f"Warning: Entropy {entropy:.3f} exceeds physical bound at bond {i}")
         self.tensors[i] = A_new

@@ -43,14 +43,25 @@ def _stream(t: torch.Tensor) -> ctypes.c_void_p:
     return ctypes.c_void_p(torch.cuda.current_stream(t.device).cuda_stream)
 
 
+def _plain(t: torch.Tensor) -> torch.Tensor:
+    """Materialise torch's lazy conjugate / negative views (e.g. the Vh that
+    torch.linalg.svd returns): the kernels read raw memory."""
+    if t.is_conj():
+        t = t.resolve_conj()
+    if t.is_neg():
+        t = t.resolve_neg()
+    return t
+
+
 def _ptr(t: torch.Tensor) -> ctypes.c_void_p:
+    assert not t.is_conj() and not t.is_neg(), "pass resolved tensors to the kernels"
     return ctypes.c_void_p(t.data_ptr())
 
 
 def gate_to_host(U, shape: Tuple[int, ...]) -> np.ndarray:
     """Gate values as interleaved float64 (re, im), row-major, on the host."""
     if isinstance(U, torch.Tensor):
-        arr = U.detach().to("cpu").numpy()
+        arr = U.detach().resolve_conj().resolve_neg().to("cpu").numpy()
     else:
         arr = np.asarray(U)
     arr = np.ascontiguousarray(arr.astype(np.complex128).reshape(shape))
@@ -71,8 +82,8 @@ def theta(A: torch.Tensor, B: torch.Tensor, gate64: np.ndarray, prec: str = "nat
         raise TypeError("theta: A and B must share a dtype")
     if A.dim() != 3 or B.dim() != 3 or A.shape[1] != 2 or B.shape[1] != 2 or A.shape[2] != B.shape[0]:
         raise ValueError(f"theta: bad shapes {tuple(A.shape)} x {tuple(B.shape)}")
-    A = A.contiguous()
-    B = B.contiguous()
+    A = _plain(A).contiguous()
+    B = _plain(B).contiguous()
     cl, cm, cr = A.shape[0], A.shape[2], B.shape[2]
     if out is None:
         out = torch.empty((2 * cl, 2 * cr), dtype=A.dtype, device=A.device)
@@ -91,7 +102,7 @@ _RANK_BYTES = ctypes.sizeof(RankResult)
 def rank_select_async(S: torch.Tensor, eps: float, cap: int) -> torch.Tensor:
     """Launch the rank rule; returns the device result buffer (uint8[48])."""
     _dev(S, "S")
-    S = S.contiguous()
+    S = _plain(S).contiguous()
     buf = torch.empty(_RANK_BYTES, dtype=torch.uint8, device=S.device)
     check(lib().aq_rank_select(_real_code(S.dtype), _ptr(S), S.numel(), float(eps), int(cap),
                                _ffi.AQ_RANK_REF, _ptr(buf), _stream(S)), "aq_rank_select")
@@ -111,6 +122,7 @@ def rank_select(S: torch.Tensor, eps: float, cap: int) -> RankResult:
 def rebuild(U: torch.Tensor, S: torch.Tensor, Vh: torch.Tensor, k: int, chiL: int, chiR: int):
     """A' = (U[:, :k] * S[:k]).reshape(chiL, 2, k), B' = Vh[:k].reshape(k, 2, chiR)."""
     _dev(U, "U")
+    U, Vh = _plain(U), _plain(Vh)
     dt = U.dtype
     rows, cols = 2 * chiL, 2 * chiR
     A = torch.empty((chiL, 2, k), dtype=dt, device=U.device)
@@ -126,7 +138,7 @@ def rebuild(U: torch.Tensor, S: torch.Tensor, Vh: torch.Tensor, k: int, chiL: in
 def apply_1q(T: torch.Tensor, gate64: np.ndarray) -> torch.Tensor:
     """T'[a,t,b] = sum_s U[s,t] T[a,s,b] (AdaptiveMPS convention)."""
     _dev(T, "T")
-    T = T.contiguous()
+    T = _plain(T).contiguous()
     out = torch.empty_like(T)
     g = np.ascontiguousarray(gate64, dtype=np.float64)
     check(lib().aq_apply_1q(_code(T.dtype), _ptr(T), _ptr(out), T.shape[0], T.shape[2],
@@ -141,6 +153,7 @@ def cgemm(A: torch.Tensor, B: torch.Tensor, conj_a: bool = False, conj_b: bool =
     _dev(A, "A")
     if A.dtype != B.dtype or A.dim() != 2 or B.dim() != 2 or A.shape[1] != B.shape[0]:
         raise ValueError(f"cgemm: bad operands {tuple(A.shape)} {A.dtype} x {tuple(B.shape)} {B.dtype}")
+    A, B = _plain(A), _plain(B)
     M, K = A.shape
     N = B.shape[1]
     if out is None:
@@ -160,7 +173,7 @@ def sample_bits(tensors: Sequence[torch.Tensor], uniforms: torch.Tensor, prec: s
     dt = tensors[0].dtype
     dev = tensors[0].device
     _dev(tensors[0], "site tensors")
-    ts = [t if (t.dtype == dt and t.is_contiguous()) else t.to(dt).contiguous() for t in tensors]
+    ts = [_plain(t).to(dt).contiguous() for t in tensors]
     chi = [1] + [int(t.shape[2]) for t in ts]
     for i, t in enumerate(ts):
         if int(t.shape[0]) != chi[i] or int(t.shape[1]) != 2:
@@ -169,7 +182,7 @@ def sample_bits(tensors: Sequence[torch.Tensor], uniforms: torch.Tensor, prec: s
         raise ValueError("sampler: last site must have right bond 1")
     S = int(uniforms.shape[0])
     assert uniforms.dtype == torch.float64 and tuple(uniforms.shape) == (S, n)
-    uniforms = uniforms.contiguous()
+    uniforms = _plain(uniforms).contiguous()
     chi_arr = (ctypes.c_int64 * (n + 1))(*chi)
     ptrs = (ctypes.c_void_p * n)(*[t.data_ptr() for t in ts])
     ws_bytes = lib().aq_sample_workspace_bytes(_code(dt), n, chi_arr, S)

@@ -1,0 +1,75 @@
+"""Live per-kernel timing with HIP events for roofline accounting.
+
+When enabled, instrumented launches record a start/end event pair on the
+stream they are launched on (torch's current stream, which is the stream the
+HIP entry points receive) together with the launch's algorithmic flops and
+bytes.  ``summary()`` reports, per kernel, the mean launch duration and the
+achieved rate against the MI355X peak of its bound.
+"""
+from __future__ import annotations
+
+import contextlib
+from collections import defaultdict
+
+import torch
+
+PEAK = {"fp64_mfma": 78.6, "fp32_mfma": 157.3, "bf16_mfma": 2500.0, "hbm": 8000.0}
+UNIT = {"fp64_mfma": "TFLOP/s", "fp32_mfma": "TFLOP/s", "bf16_mfma": "TFLOP/s", "hbm": "GB/s"}
+
+_enabled = False
+_records = []
+
+
+def enable(on: bool) -> None:
+    global _enabled
+    _enabled = on
+
+
+def reset() -> None:
+    _records.clear()
+
+
+@contextlib.contextmanager
+def timed(name: str, work: float, bound: str, nbytes: float = 0.0):
+    """Time one launch.  work: algorithmic flops (MFMA bounds) or bytes (hbm)."""
+    if not _enabled:
+        yield
+        return
+    s = torch.cuda.Event(enable_timing=True)
+    e = torch.cuda.Event(enable_timing=True)
+    s.record()
+    yield
+    e.record()
+    _records.append((name, s, e, float(work), bound, float(nbytes)))
+
+
+def summary():
+    if not _records:
+        return None
+    torch.cuda.synchronize()
+    agg = defaultdict(lambda: [0, 0.0, 0.0, None, 0.0])
+    for name, s, e, work, bound, nb in _records:
+        a = agg[name]
+        a[0] += 1
+        a[1] += s.elapsed_time(e)
+        a[2] += work
+        a[3] = bound
+        a[4] += nb
+    kernels = {}
+    for name, (cnt, ms, work, bound, nb) in agg.items():
+        mean_ms = ms / cnt
+        per = work / cnt
+        if bound == "hbm":
+            achieved = per / (mean_ms * 1e-3) / 1e9
+        else:
+            achieved = per / (mean_ms * 1e-3) / 1e12
+        kernels[name] = {"launches": cnt, "mean_ms": mean_ms, "work_per_launch": per,
+                         "bound": bound, "achieved": achieved, "peak": PEAK[bound],
+                         "unit": UNIT[bound], "frac": achieved / PEAK[bound],
+                         "total_ms": ms}
+    dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
+    d = kernels[dom]
+    return {"kernel": dom, "bound": "mfma" if d["bound"] != "hbm" else "hbm",
+            "achieved": d["achieved"], "peak": d["peak"], "unit": d["unit"],
+            "frac": d["frac"], "traffic": None, "mean_launch_ms": d["mean_ms"],
+            "all": kernels}

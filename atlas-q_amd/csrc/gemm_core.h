@@ -39,6 +39,9 @@ template <> struct Mfma<float> {
   // operand lane layout: lane holds A[op_idx][op_k] / B[op_k][op_idx]
   static __device__ __forceinline__ int op_idx(int lane) { return lane & 31; }
   static __device__ __forceinline__ int op_k(int lane) { return lane >> 5; }
+  // tile row fed to MFMA row op_idx(lane): identity (rows 2l, 2l+1 already land
+  // in accumulator registers r, r+1)
+  static __device__ __forceinline__ int a_row(int lane) { return lane & 31; }
   // accumulator layout: register r of lane -> (out_row, out_col)
   static __device__ __forceinline__ int out_row(int lane, int r) {
     return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
@@ -57,7 +60,17 @@ template <> struct Mfma<double> {
   }
   static __device__ __forceinline__ int op_idx(int lane) { return lane & 15; }
   static __device__ __forceinline__ int op_k(int lane) { return lane >> 4; }
-  static __device__ __forceinline__ int out_row(int lane, int r) { return 4 * (lane >> 4) + r; }
+  // The FP64 instruction returns MFMA row 4r + lane/16 in register r.  Feeding
+  // tile row pi(i) = 8(r>>1) + 2g + (r&1) (i = 4r + g) to MFMA row i puts tile
+  // rows (2l, 2l+1) in registers (r, r+1) of one lane, as the FP32 layout does,
+  // so the pairwise epilogues (gate, outcome halves) are shared.
+  static __device__ __forceinline__ int a_row(int lane) {
+    const int i = lane & 15, r = i >> 2, g = i & 3;
+    return 8 * (r >> 1) + 2 * g + (r & 1);
+  }
+  static __device__ __forceinline__ int out_row(int lane, int r) {
+    return 8 * (r >> 1) + 2 * (lane >> 4) + (r & 1);
+  }
   static __device__ __forceinline__ int out_col(int lane) { return lane & 15; }
   static constexpr int COL_LANES = 16;
 };
@@ -243,7 +256,7 @@ __device__ __forceinline__ void cgemm_mainloop(const AOp& aop, const BOp& bop, c
       T ar[Frag<T>::RB], ai[Frag<T>::RB], nai[Frag<T>::RB];
 #pragma unroll
       for (int rb = 0; rb < Frag<T>::RB; ++rb) {
-        const int row = 32 * wm + rb * E + M::op_idx(lane);
+        const int row = 32 * wm + rb * E + M::a_row(lane);
         ar[rb] = sm.a[0][kr][row];
         ai[rb] = sm.a[1][kr][row];
         nai[rb] = -ai[rb];

@@ -1,0 +1,232 @@
+// Batched two-sided Jacobi eigensolver for small Hermitian FP64 matrices
+// (n <= 64), one work-group per matrix, everything resident in LDS.
+//
+// Cyclic Jacobi with the round-robin (tournament) ordering: each round
+// rotates n/2 disjoint index pairs at once; a sweep is n-1 rounds.  For a
+// pair (p, q) with a = A_pp, b = A_qq, z = A_pq = |z| e^{i phi}:
+//   tau = (b - a) / (2|z|),  t = sign(tau) / (|tau| + sqrt(1 + tau^2)),
+//   c = 1/sqrt(1+t^2), s = t c,  J = [[c, s], [-s e^{-i phi}, c e^{-i phi}]]
+// and A <- J^H A J, V <- V J.  The off-diagonal Frobenius norm is reduced
+// across the work-group after every sweep; iteration stops when it falls
+// below tol * ||A||_F or after max_sweeps.  Eigenvalues are returned in
+// descending order with their eigenvectors (columns of V).
+//
+// Used for the projected cores of the randomized SVD, the Gram matrices of
+// small two-site updates, the sketch spectra of the rank-predictor features
+// and the batched small updates.
+#include <math.h>
+
+#include "aq_common.h"
+
+namespace aq {
+
+constexpr int kJmax = 64;
+
+struct JacobiSmem {
+  double2 a[kJmax][kJmax + 1];
+  double2 v[kJmax][kJmax + 1];
+  double cs[kJmax / 2];       // c per pair
+  double2 sp[kJmax / 2];      // s e^{-i phi} per pair
+  double2 ep[kJmax / 2];      // e^{-i phi} per pair
+  int pp[kJmax / 2], pq[kJmax / 2];
+  int perm[kJmax];
+  double red[256];
+  double lam[kJmax];
+  int order[kJmax];
+};
+
+__device__ __forceinline__ double2 cm(double2 x, double2 y) {
+  return make_double2(x.x * y.x - x.y * y.y, x.x * y.y + x.y * y.x);
+}
+__device__ __forceinline__ double2 cconj(double2 x) { return make_double2(x.x, -x.y); }
+__device__ __forceinline__ double2 cadd2(double2 x, double2 y) { return make_double2(x.x + y.x, x.y + y.y); }
+__device__ __forceinline__ double2 cscale(double2 x, double s) { return make_double2(x.x * s, x.y * s); }
+
+__device__ double block_sum(double v, double* red) {
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  const double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+__global__ __launch_bounds__(256) void heev_jacobi_kernel(int n, const double2* __restrict__ A,
+                                                          int64_t lda, int64_t sA,
+                                                          double* __restrict__ W,
+                                                          double2* __restrict__ V, int64_t ldv,
+                                                          int64_t sV, int max_sweeps, double tol,
+                                                          int* __restrict__ sweeps_out) {
+  extern __shared__ unsigned char smraw[];
+  JacobiSmem& sm = *reinterpret_cast<JacobiSmem*>(smraw);
+  const int tid = threadIdx.x;
+  const int64_t b = blockIdx.x;
+  const int ne = n + (n & 1);  // even order for the tournament (padding index = n)
+  const double2* Ab = A + b * sA;
+  for (int i = tid; i < ne * ne; i += blockDim.x) {
+    const int r = i / ne, c = i % ne;
+    double2 x = make_double2(0.0, 0.0);
+    if (r < n && c < n) x = Ab[(int64_t)r * lda + c];
+    sm.a[r][c] = x;
+    sm.v[r][c] = make_double2(r == c ? 1.0 : 0.0, 0.0);
+  }
+  for (int i = tid; i < ne; i += blockDim.x) sm.perm[i] = i;
+  __syncthreads();
+  // symmetrize (the input is Hermitian up to rounding)
+  for (int i = tid; i < ne * ne; i += blockDim.x) {
+    const int r = i / ne, c = i % ne;
+    if (r < c) {
+      const double2 u = sm.a[r][c], l = cconj(sm.a[c][r]);
+      const double2 h = make_double2(0.5 * (u.x + l.x), 0.5 * (u.y + l.y));
+      sm.a[r][c] = h;
+      sm.a[c][r] = cconj(h);
+    } else if (r == c) {
+      sm.a[r][c].y = 0.0;
+    }
+  }
+  __syncthreads();
+  double fro = 0.0;
+  for (int i = tid; i < ne * ne; i += blockDim.x) {
+    const double2 x = sm.a[i / ne][i % ne];
+    fro += x.x * x.x + x.y * x.y;
+  }
+  fro = block_sum(fro, sm.red);
+  const double thresh2 = tol * tol * fro;
+  int sweep = 0;
+  const int half = ne / 2;
+  for (; sweep < max_sweeps; ++sweep) {
+    double off = 0.0;
+    for (int i = tid; i < ne * ne; i += blockDim.x) {
+      const int r = i / ne, c = i % ne;
+      if (r != c) {
+        const double2 x = sm.a[r][c];
+        off += x.x * x.x + x.y * x.y;
+      }
+    }
+    off = block_sum(off, sm.red);
+    if (off <= thresh2 || ne < 2) break;
+    for (int round = 0; round < ne - 1; ++round) {
+      // pairs of this round: (perm[k], perm[ne-1-k])
+      if (tid < half) {
+        int p = sm.perm[tid], q = sm.perm[ne - 1 - tid];
+        if (p > q) { const int t = p; p = q; q = t; }
+        sm.pp[tid] = p;
+        sm.pq[tid] = q;
+        const double a = sm.a[p][p].x, bq = sm.a[q][q].x;
+        const double2 z = sm.a[p][q];
+        const double az = sqrt(z.x * z.x + z.y * z.y);
+        double c = 1.0, s = 0.0;
+        double2 e = make_double2(1.0, 0.0);
+        if (az > 1e-300 * (fabs(a) + fabs(bq) + 1e-300) && az > 0.0) {
+          const double tau = (bq - a) / (2.0 * az);
+          const double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+          c = 1.0 / sqrt(1.0 + t * t);
+          s = t * c;
+          e = make_double2(z.x / az, -z.y / az);  // e^{-i phi}
+        }
+        sm.cs[tid] = c;
+        sm.ep[tid] = e;
+        sm.sp[tid] = cscale(e, s);
+      }
+      __syncthreads();
+      // columns: A <- A J, V <- V J   (item = row r, pair k)
+      for (int it = tid; it < ne * half; it += blockDim.x) {
+        const int k = it / ne, r = it % ne;
+        const int p = sm.pp[k], q = sm.pq[k];
+        const double c = sm.cs[k];
+        const double s = sm.sp[k].x * sm.ep[k].x + sm.sp[k].y * sm.ep[k].y;  // |s|, sign kept
+        const double2 e = sm.ep[k];
+        const double2 se = sm.sp[k];
+        const double2 ap = sm.a[r][p], aq = sm.a[r][q];
+        // new p = c ap - s e aq ; new q = s ap + c e aq
+        sm.a[r][p] = cadd2(cscale(ap, c), cscale(cm(se, aq), -1.0));
+        sm.a[r][q] = cadd2(cscale(ap, s), cscale(cm(e, aq), c));
+        const double2 vp = sm.v[r][p], vq = sm.v[r][q];
+        sm.v[r][p] = cadd2(cscale(vp, c), cscale(cm(se, vq), -1.0));
+        sm.v[r][q] = cadd2(cscale(vp, s), cscale(cm(e, vq), c));
+      }
+      __syncthreads();
+      // rows: A <- J^H A   (item = pair k, column r)
+      for (int it = tid; it < ne * half; it += blockDim.x) {
+        const int k = it / ne, r = it % ne;
+        const int p = sm.pp[k], q = sm.pq[k];
+        const double c = sm.cs[k];
+        const double s = sm.sp[k].x * sm.ep[k].x + sm.sp[k].y * sm.ep[k].y;
+        const double2 ec = cconj(sm.ep[k]);
+        const double2 sec = cconj(sm.sp[k]);
+        const double2 ap = sm.a[p][r], aq = sm.a[q][r];
+        // new p = c ap - s conj(e) aq ; new q = s ap + c conj(e) aq
+        sm.a[p][r] = cadd2(cscale(ap, c), cscale(cm(sec, aq), -1.0));
+        sm.a[q][r] = cadd2(cscale(ap, s), cscale(cm(ec, aq), c));
+      }
+      __syncthreads();
+      // the rotated pairs are exactly diagonalised: clean the rounding
+      if (tid < half) {
+        const int p = sm.pp[tid], q = sm.pq[tid];
+        sm.a[p][q] = make_double2(0.0, 0.0);
+        sm.a[q][p] = make_double2(0.0, 0.0);
+        sm.a[p][p].y = 0.0;
+        sm.a[q][q].y = 0.0;
+      }
+      // tournament rotation: perm[0] fixed, the rest rotate by one
+      __syncthreads();
+      if (tid == 0) {
+        const int last = sm.perm[ne - 1];
+        for (int i = ne - 1; i > 1; --i) sm.perm[i] = sm.perm[i - 1];
+        sm.perm[1] = last;
+      }
+      __syncthreads();
+    }
+  }
+  // eigenvalues, descending order by rank (ties broken by index)
+  for (int i = tid; i < n; i += blockDim.x) sm.lam[i] = sm.a[i][i].x;
+  __syncthreads();
+  for (int i = tid; i < n; i += blockDim.x) {
+    const double li = sm.lam[i];
+    int rank = 0;
+    for (int j = 0; j < n; ++j) {
+      const double lj = sm.lam[j];
+      rank += (lj > li) || (lj == li && j < i);
+    }
+    sm.order[rank] = i;
+  }
+  __syncthreads();
+  double* Wb = W + b * n;
+  double2* Vb = V + b * sV;
+  for (int i = tid; i < n; i += blockDim.x) Wb[i] = sm.lam[sm.order[i]];
+  for (int i = tid; i < n * n; i += blockDim.x) {
+    const int r = i / n, c = i % n;
+    Vb[(int64_t)r * ldv + c] = sm.v[r][sm.order[c]];
+  }
+  if (tid == 0 && sweeps_out) sweeps_out[b] = sweep;
+}
+
+static bool g_jac_attr = false;
+
+}  // namespace aq
+
+using namespace aq;
+
+extern "C" int aq_heev_small(int64_t n, const void* A, int64_t lda, int64_t strideA, double* W,
+                             void* V, int64_t ldv, int64_t strideV, int64_t batch, int max_sweeps,
+                             double tol, int* sweeps, hipStream_t stream) {
+  AQ_REQUIRE(A && W && V, AQ_ERR_ARG, "aq_heev_small: null pointer");
+  AQ_REQUIRE(n >= 1 && n <= kJmax, AQ_ERR_ARG, "aq_heev_small: n must be in [1, %d] (got %lld)",
+             kJmax, (long long)n);
+  AQ_REQUIRE(batch >= 1 && batch < (1LL << 31), AQ_ERR_ARG, "aq_heev_small: bad batch");
+  AQ_REQUIRE(lda >= n && ldv >= n, AQ_ERR_ARG, "aq_heev_small: leading dimension < n");
+  AQ_REQUIRE(max_sweeps >= 1 && max_sweeps <= 100, AQ_ERR_ARG, "aq_heev_small: max_sweeps");
+  if (!g_jac_attr) {
+    AQ_HIP(hipFuncSetAttribute((const void*)heev_jacobi_kernel,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, sizeof(JacobiSmem)));
+    g_jac_attr = true;
+  }
+  heev_jacobi_kernel<<<(unsigned)batch, 256, sizeof(JacobiSmem), stream>>>(
+      (int)n, (const double2*)A, lda, strideA, W, (double2*)V, ldv, strideV, max_sweeps, tol,
+      sweeps);
+  AQ_LAUNCH_CHECK();
+  return AQ_OK;
+}

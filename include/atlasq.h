@@ -124,6 +124,42 @@ int aq_sample(int dtype, int prec, int flags, int64_t n, const void* const* site
               int64_t shots, const double* uniforms, uint64_t* bits, double* p0_out,
               void* workspace, int64_t workspace_bytes, hipStream_t stream);
 
+/* Small Hermitian eigensolver (no reference counterpart: the reference calls
+ * LAPACK/cuSOLVER SVD, linalg_robust.py:17-52).  Batched two-sided Jacobi in
+ * FP64, one work-group per n x n matrix (n <= 64) held in LDS.  A: complex128,
+ * matrix b at A + b*strideA, row-major with leading dimension lda (the input
+ * is symmetrised).  W: float64 [batch, n] eigenvalues in descending order;
+ * V: complex128 eigenvectors as columns, matrix b at V + b*strideV.  Stops
+ * when the off-diagonal norm falls below tol*||A||_F or after max_sweeps;
+ * sweeps (nullable, int[batch]) receives the sweep count. */
+int aq_heev_small(int64_t n, const void* A, int64_t lda, int64_t strideA, double* W, void* V,
+                  int64_t ldv, int64_t strideV, int64_t batch, int max_sweeps, double tol,
+                  int* sweeps, hipStream_t stream);
+
+/* Rank-predictor MLP 32 -> 128 -> 128 -> 1 with GELU and BF16 weights (no
+ * reference counterpart for pre-SVD prediction; the reference's predictors,
+ * src/atlas_q/tools_qih/ai_rank_predictor.py:14-89 and
+ * finetuned_predictor.py:8-23, run after a full SVD at tn_core.py:294, and its
+ * training recipe, ai_rank_predictor.py:117-195, is AdamW on an MSE loss).
+ * Parameters: one FP32 master vector of aq_mlp_num_params() floats laid out
+ * W1[128][32] | b1[128] | W2[128][128] | b2[128] | W3[128] | b3, and a BF16
+ * copy (uint16) of the same length used by forward/backward for the weights.
+ * X: float32 [batch, 32] (row-major), y: float32 [batch]. */
+int aq_mlp_num_params(void);
+int aq_mlp_sync_bf16(const float* master, uint16_t* wb, hipStream_t stream);
+int aq_mlp_forward(const float* X, int64_t batch, const uint16_t* wb, const float* master,
+                   float* y, hipStream_t stream);
+/* Gradient of sum_s dy[s] * y[s]; workspace: aq_mlp_workspace_floats(batch)
+ * floats (per-block partials, summed in a fixed order). */
+int64_t aq_mlp_workspace_floats(int64_t batch);
+int aq_mlp_backward(const float* X, const float* dy, int64_t batch, const uint16_t* wb,
+                    const float* master, float* grad, float* workspace, int64_t workspace_floats,
+                    hipStream_t stream);
+/* AdamW step on the master (decoupled decay on weights only), refreshing wb. */
+int aq_mlp_adamw(float* master, const float* grad, float* m, float* v, uint16_t* wb, double lr,
+                 double beta1, double beta2, double eps, double wd, double grad_scale, int step,
+                 hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -107,7 +107,7 @@ def _spectra():
 def test_rank_rule_matches_oracle(dt):
     n = 0
     for S in _spectra():
-        St = torch.tensor(S, dtype=dt)
+        St = torch.tensor(np.ascontiguousarray(S), dtype=dt)
         for eps in (1e-1, 1e-2, 1e-5, 1e-8, 1e-10):
             for cap in (1, 3, len(S) // 2 + 1, 10 ** 6):
                 ref = R.choose_rank_from_sigma(St, eps, cap)
