@@ -25,6 +25,24 @@ c_vp = ctypes.c_void_p
 c_dp = ctypes.POINTER(ctypes.c_double)
 
 
+import numpy as _np
+
+GEMM_DESC = _np.dtype([("A", "u8"), ("B", "u8"), ("C", "u8"), ("M", "i8"), ("N", "i8"),
+                       ("K", "i8"), ("a_s0", "i8"), ("a_s1", "i8"), ("b_s0", "i8"),
+                       ("b_s1", "i8"), ("c_s0", "i8"), ("c_s1", "i8"), ("conj_a", "i4"),
+                       ("conj_b", "i4"), ("kind", "i4"), ("pad", "i4"), ("gate", "f8", (32,))])
+SITE1Q_DESC = _np.dtype([("T_in", "u8"), ("T_out", "u8"), ("chiL", "i8"), ("chiR", "i8"),
+                         ("gate", "f8", (8,))])
+SAMPLE_DESC = _np.dtype([("L", "u8"), ("off", "u8"), ("site", "u8"), ("Tout", "u8"),
+                         ("part", "u8"), ("uniforms", "u8"), ("bits", "u8"),
+                         ("off_next", "u8"), ("ldL", "i8"), ("S", "i8"), ("chiL", "i8"),
+                         ("chiR", "i8"), ("n", "i8"), ("site_index", "i8"), ("words", "i8")])
+COPY_DESC = _np.dtype([("src", "u8"), ("dst", "u8"), ("row_scale", "u8"), ("col_scale", "u8"),
+                       ("rows", "i8"), ("cols", "i8"), ("s0", "i8"), ("s1", "i8"),
+                       ("conj", "i4"), ("pad", "i4")])
+AQ_GROUPED_GEMM, AQ_GROUPED_THETA = 0, 1
+
+
 class RankResult(ctypes.Structure):
     _fields_ = [("k", ctypes.c_int32), ("flags", ctypes.c_int32),
                 ("eps_local", ctypes.c_double), ("entropy", ctypes.c_double),
@@ -56,6 +74,14 @@ SIGNATURES = {
     "aq_mlp_adamw": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, ctypes.c_double, ctypes.c_double,
                              ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                              c_int, c_vp]),
+    "aq_desc_sizes": (c_int, [c_vp]),
+    "aq_grouped_gemm": (c_int, [c_int, c_vp, c_vp, c_int, c_i64, c_int, c_int, c_vp]),
+    "aq_grouped_1q": (c_int, [c_vp, c_vp, c_int, c_i64, c_int, c_vp]),
+    "aq_rank_select_batched": (c_int, [c_int, c_vp, c_vp, c_vp, ctypes.c_double, c_int, c_vp,
+                                       c_vp]),
+    "aq_sample_grouped_site": (c_int, [c_vp, c_vp, c_vp, c_int, c_i64, c_i64, c_int, c_vp]),
+    "aq_zero_u64": (c_int, [c_vp, c_i64, c_vp]),
+    "aq_grouped_copy": (c_int, [c_vp, c_vp, c_int, c_i64, c_int, c_vp]),
 }
 
 _lib = None
@@ -81,6 +107,16 @@ def lib() -> ctypes.CDLL:
             fn.argtypes = args
         _lib = L
     return _lib
+
+
+def check_layouts() -> None:
+    """Assert that the NumPy descriptor layouts match the library's structs."""
+    out = (ctypes.c_int64 * 5)()
+    check(lib().aq_desc_sizes(out), "aq_desc_sizes")
+    want = [GEMM_DESC.itemsize, SITE1Q_DESC.itemsize, SAMPLE_DESC.itemsize,
+            COPY_DESC.itemsize, ctypes.sizeof(RankResult)]
+    if list(out) != want:
+        raise RuntimeError(f"descriptor layout mismatch: library {list(out)} vs binding {want}")
 
 
 def check(status: int, what: str) -> None:

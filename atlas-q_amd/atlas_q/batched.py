@@ -1,0 +1,405 @@
+"""Many independent MPS advanced together by grouped persistent launches.
+
+Storage: every site tensor of every MPS lives in one complex64 arena in HBM,
+in a slot sized for the largest bond dimensions that site can reach
+(min(chi_max, 2^q, 2^(n-q)) on the left, likewise on the right), so updates
+happen in place and nothing is reallocated per gate; the current bond
+dimensions are tracked on the host.  A circuit layer is executed as
+
+  single-qubit gates  : 1 grouped launch for all sites of all MPS (in place)
+  two-qubit sub-layer : grouped theta (MFMA, gate in the epilogue)
+                        -> FP64 Gram of the smaller side (grouped GEMM)
+                        -> eigen-decomposition per size bucket (Jacobi kernel
+                           for orders <= 64, rocSOLVER batched above)
+                        -> batched rank rule (one launch, one 48-byte-per-gate
+                           device->host copy: the only sync of the sub-layer)
+                        -> grouped projection GEMM + grouped rebuild into the
+                           slots
+
+Each gate follows AdaptiveMPS.apply_two_site_gate (adaptive_mps.py:182-308):
+"U" gate convention, the rank rule with per-bond caps, S absorbed left.
+Single-qubit gates follow adaptive_mps.py:177 (U^T).  The sampler runs the
+sweep of mps_pytorch.py:247-355 for all MPS with two grouped launches per
+site index and can be captured into a HIP graph.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from typing import List, Sequence, Tuple
+
+import numpy as np
+import torch
+
+from . import _ffi
+from ._ffi import (COPY_DESC, GEMM_DESC, SAMPLE_DESC, SITE1Q_DESC, RankResult, check, lib)
+from .autotune import Autotuner
+from .eig import eigh_desc
+
+C64 = torch.complex64
+C128 = torch.complex128
+
+
+def _vp(x) -> ctypes.c_void_p:
+    return ctypes.c_void_p(x if isinstance(x, int) else x.data_ptr())
+
+
+def _stream() -> ctypes.c_void_p:
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _upload(arr: np.ndarray, device) -> torch.Tensor:
+    return torch.from_numpy(np.ascontiguousarray(arr).view(np.uint8).copy()).to(device)
+
+
+def _prefix(counts: np.ndarray) -> Tuple[np.ndarray, int]:
+    pre = np.zeros(len(counts), np.int64)
+    if len(counts) > 1:
+        pre[1:] = np.cumsum(counts[:-1])
+    return pre, int(counts.sum())
+
+
+def _cdiv(a, b):
+    return -(-a // b)
+
+
+class BatchedMPS:
+    """Population of MPS (n_j qubits, cap chi_j) starting in |0...0>."""
+
+    def __init__(self, specs: Sequence[Tuple[int, int]], eps_bond: float = 1e-8,
+                 device="cuda", autotuner: Autotuner = None):
+        self.dev = torch.device(device)
+        if self.dev.type != "cuda":
+            raise RuntimeError("BatchedMPS runs on a HIP device only")
+        _ffi.check_layouts()
+        self.n = np.array([s[0] for s in specs], np.int64)
+        self.cap = np.array([s[1] for s in specs], np.int64)
+        self.eps = float(eps_bond)
+        self.M = len(specs)
+        self.nmax = int(self.n.max())
+        # bond capacities: bcap[j, q] for bond q (left of site q), q = 0..n_j
+        self.bcap = np.zeros((self.M, self.nmax + 1), np.int64)
+        for j in range(self.M):
+            q = np.arange(self.n[j] + 1)
+            e = np.minimum(q, self.n[j] - q)
+            self.bcap[j, : self.n[j] + 1] = np.minimum(self.cap[j], 2.0 ** np.minimum(e, 62))
+        self.slot_off = np.zeros((self.M, self.nmax), np.int64)   # in complex elements
+        off = 0
+        for j in range(self.M):
+            for q in range(self.n[j]):
+                self.slot_off[j, q] = off
+                off += 2 * self.bcap[j, q] * self.bcap[j, q + 1]
+        self.arena = torch.zeros(off, dtype=C64, device=self.dev)
+        self.dims = np.ones((self.M, self.nmax + 1), np.int64)
+        # |0...0>: every site [1, 2, 1] = (1, 0)
+        first = torch.from_numpy(np.concatenate([self.slot_off[j, : self.n[j]]
+                                                 for j in range(self.M)])).to(self.dev)
+        self.arena[first] = 1.0
+        self.tuner = autotuner if autotuner is not None else Autotuner.default()
+        self.logs = {"mps": [], "bond": [], "k": [], "eps_local": [], "entropy": []}
+        self._base = self.arena.data_ptr()
+
+    # ------------------------------------------------------------ views
+    def site(self, j: int, q: int) -> torch.Tensor:
+        dl, dr = int(self.dims[j, q]), int(self.dims[j, q + 1])
+        o = int(self.slot_off[j, q])
+        return self.arena[o: o + dl * 2 * dr].view(dl, 2, dr)
+
+    def tensors(self, j: int) -> List[torch.Tensor]:
+        return [self.site(j, q) for q in range(int(self.n[j]))]
+
+    def bond_dims(self, j: int) -> List[int]:
+        return [int(x) for x in self.dims[j, 1: self.n[j]]]
+
+    def _ptr(self, j, q) -> int:
+        return self._base + 8 * int(self.slot_off[j, q])
+
+    # ------------------------------------------------------------ 1q layer
+    def apply_1q_layer(self, ops: Sequence[Sequence[Tuple[int, np.ndarray]]]):
+        """ops[j] = [(q, U2), ...]; at most one gate per site per call."""
+        rows = [(j, q, U) for j, lst in enumerate(ops) for q, U in lst]
+        if not rows:
+            return
+        d = np.zeros(len(rows), SITE1Q_DESC)
+        for i, (j, q, U) in enumerate(rows):
+            p = self._ptr(j, q)
+            d[i]["T_in"] = p
+            d[i]["T_out"] = p
+            d[i]["chiL"] = self.dims[j, q]
+            d[i]["chiR"] = self.dims[j, q + 1]
+            d[i]["gate"] = np.asarray(U, np.complex128).reshape(-1).view(np.float64)
+        cnt = d["chiL"] * d["chiR"]
+        pre, total = _prefix(cnt)
+        dd, pd = _upload(d, self.dev), torch.from_numpy(pre).to(self.dev)
+        grid = self.tuner.grid("elem", total, None)   # in place: no timing trials
+        check(lib().aq_grouped_1q(_vp(dd), _vp(pd), len(d), total, grid, _stream()),
+              "aq_grouped_1q")
+
+    # ------------------------------------------------------------ 2q layer
+    def _grouped_gemm(self, dtype_code, descs: np.ndarray, layout: int, key: str):
+        if len(descs) == 0:
+            return
+        paired = descs["kind"] == _ffi.AQ_GROUPED_THETA
+        tn = np.where(paired, _cdiv(descs["N"] // 2, 64), _cdiv(descs["N"], 128))
+        tiles = _cdiv(descs["M"], 64) * tn
+        pre, total = _prefix(tiles)
+        dd, pd = _upload(descs, self.dev), torch.from_numpy(pre).to(self.dev)
+
+        def run(g):
+            check(lib().aq_grouped_gemm(dtype_code, _vp(dd), _vp(pd), len(descs), total, layout,
+                                        g, _stream()), "aq_grouped_gemm")
+        run(self.tuner.grid(key, total, run))
+
+    def apply_2q_layer(self, ops: Sequence[Sequence[Tuple[int, np.ndarray]]]):
+        """ops[j] = [(bond, U4), ...] with disjoint bonds within each MPS."""
+        gates = [(j, b, np.asarray(U, np.complex128).reshape(4, 4))
+                 for j, lst in enumerate(ops) for b, U in lst]
+        G = len(gates)
+        if G == 0:
+            return
+        jj = np.array([g[0] for g in gates], np.int64)
+        bb = np.array([g[1] for g in gates], np.int64)
+        dl = self.dims[jj, bb]
+        dm = self.dims[jj, bb + 1]
+        dr = self.dims[jj, bb + 2]
+        m, n = 2 * dl, 2 * dr
+        # ---- theta into the X arena (complex64)
+        xoff, xtot = _prefix(m * n)
+        X = torch.empty(xtot, dtype=C64, device=self.dev)
+        th = np.zeros(G, GEMM_DESC)
+        th["A"] = [self._ptr(j, b) for j, b in zip(jj, bb)]
+        th["B"] = [self._ptr(j, b + 1) for j, b in zip(jj, bb)]
+        th["C"] = X.data_ptr() + 8 * xoff
+        th["M"], th["N"], th["K"] = m, n, dm
+        th["a_s0"], th["a_s1"] = dm, 1
+        th["b_s0"], th["b_s1"] = n, 1
+        th["c_s0"], th["c_s1"] = n, 1
+        th["kind"] = _ffi.AQ_GROUPED_THETA
+        th["gate"] = np.stack([g[2].reshape(-1).view(np.float64) for g in gates])
+        self._grouped_gemm(_ffi.AQ_C64, th, 3, "theta")
+        X64 = X.to(C128)
+        x64 = X64.data_ptr() + 16 * xoff
+        # ---- FP64 Gram of the smaller side into size buckets
+        s = np.minimum(m, n)
+        left = m <= n
+        bsize = np.maximum(2, 2 ** np.ceil(np.log2(s)).astype(np.int64))
+        buckets = {}
+        for sz in np.unique(bsize):
+            idx = np.nonzero(bsize == sz)[0]
+            buckets[int(sz)] = (idx, torch.zeros((len(idx), int(sz), int(sz)), dtype=C128,
+                                                 device=self.dev))
+        gram = np.zeros(G, GEMM_DESC)
+        gptr = np.zeros(G, np.int64)
+        gld = np.zeros(G, np.int64)
+        for sz, (idx, T) in buckets.items():
+            gptr[idx] = T.data_ptr() + 16 * sz * sz * np.arange(len(idx))
+            gld[idx] = sz
+        gram["A"] = x64
+        gram["B"] = x64
+        gram["C"] = gptr
+        gram["M"] = s
+        gram["N"] = s
+        gram["K"] = np.where(left, n, m)
+        # left: X X^H  (A = X: (i,k) at i*n + k; B = X^H: (k,j) = conj X[j,k] at j*n + k)
+        # right: X^H X (A = X^H: (i,k) = conj X[k,i] at k*n + i; B = X: (k,j) at k*n + j)
+        gram["a_s0"] = np.where(left, n, 1)
+        gram["a_s1"] = np.where(left, 1, n)
+        gram["conj_a"] = np.where(left, 0, 1)
+        gram["b_s0"] = np.where(left, 1, n)
+        gram["b_s1"] = np.where(left, n, 1)
+        gram["conj_b"] = np.where(left, 1, 0)
+        gram["c_s0"], gram["c_s1"] = gld, 1
+        gram["kind"] = _ffi.AQ_GROUPED_GEMM
+        self._grouped_gemm(_ffi.AQ_C128, gram[left], 1, "gram")
+        self._grouped_gemm(_ffi.AQ_C128, gram[~left], 2, "gram")
+        # ---- eigen-decompositions per bucket (descending)
+        W_all, V_all = {}, {}
+        for sz, (idx, T) in buckets.items():
+            W, V = eigh_desc(T)
+            W_all[sz], V_all[sz] = W.contiguous(), V.contiguous()
+        # ---- batched rank rule on the float32 singular values
+        Sflat = torch.empty(int(s.sum()), dtype=torch.float32, device=self.dev)
+        soff, _ = _prefix(s)
+        Sf64 = {}
+        for sz, (idx, T) in buckets.items():
+            sv = torch.sqrt(torch.clamp(W_all[sz], min=0.0))       # [B, sz] float64
+            Sf64[sz] = sv
+            # scatter the first s_g values of each row into Sflat
+            lens = s[idx]
+            rows = np.repeat(np.arange(len(idx)), lens)
+            cols = np.concatenate([np.arange(L) for L in lens])
+            dst = np.concatenate([soff[g] + np.arange(s[g]) for g in idx])
+            Sflat[torch.from_numpy(dst).to(self.dev)] = sv[torch.from_numpy(rows).to(self.dev),
+                                                           torch.from_numpy(cols).to(self.dev)].float()
+        sp = torch.from_numpy((Sflat.data_ptr() + 4 * soff).astype(np.int64)).to(self.dev)
+        md = torch.from_numpy(s.astype(np.int64)).to(self.dev)
+        cd = torch.from_numpy(self.cap[jj].astype(np.int64)).to(self.dev)
+        res = torch.empty(G * ctypes.sizeof(RankResult), dtype=torch.uint8, device=self.dev)
+        check(lib().aq_rank_select_batched(_ffi.AQ_C64, _vp(sp), _vp(md), _vp(cd), self.eps, G,
+                                           _vp(res), _stream()), "aq_rank_select_batched")
+        rr = np.frombuffer(res.cpu().numpy().tobytes(), dtype=np.dtype(
+            [("k", "i4"), ("flags", "i4"), ("eps_local", "f8"), ("entropy", "f8"),
+             ("condS", "f8"), ("total", "f8"), ("kept", "f8")]))
+        k = rr["k"].astype(np.int64)
+        # ---- projections (FP64) and rebuild into the slots
+        vptr = np.zeros(G, np.int64)
+        sptr = np.zeros(G, np.int64)
+        for sz, (idx, T) in buckets.items():
+            vptr[idx] = V_all[sz].data_ptr() + 16 * sz * sz * np.arange(len(idx))
+            sptr[idx] = Sf64[sz].data_ptr() + 8 * sz * np.arange(len(idx))
+        inv = {sz: torch.where(v > 0, 1.0 / v, torch.zeros_like(v)) for sz, v in Sf64.items()}
+        iptr = np.zeros(G, np.int64)
+        for sz, (idx, T) in buckets.items():
+            iptr[idx] = inv[sz].data_ptr() + 8 * sz * np.arange(len(idx))
+        pcount = np.where(left, k * n, m * k)
+        poff, ptot = _prefix(pcount)
+        P = torch.empty(max(ptot, 1), dtype=C128, device=self.dev)
+        pj = np.zeros(G, GEMM_DESC)
+        pj["kind"] = _ffi.AQ_GROUPED_GEMM
+        pj["C"] = P.data_ptr() + 16 * poff
+        # left: Vh_k = U_k^H X : A(i,r) = conj V[r,i] (V row-major, ld = bucket size)
+        pj["A"] = np.where(left, vptr, x64)
+        pj["M"] = np.where(left, k, m)
+        pj["N"] = np.where(left, n, k)
+        pj["K"] = np.where(left, m, n)
+        pj["a_s0"] = np.where(left, 1, n)
+        pj["a_s1"] = np.where(left, gld, 1)
+        pj["conj_a"] = np.where(left, 1, 0)
+        # right: U_k S_k = X V_k : B(r,c) = V[r,c]
+        pj["B"] = np.where(left, x64, vptr)
+        pj["b_s0"] = np.where(left, n, gld)
+        pj["b_s1"] = 1
+        pj["c_s0"] = np.where(left, n, k)
+        pj["c_s1"] = 1
+        self._grouped_gemm(_ffi.AQ_C128, pj[left], 2, "proj")
+        self._grouped_gemm(_ffi.AQ_C128, pj[~left], 3, "proj")
+        # new shapes: site b -> [dl, 2, k], site b+1 -> [k, 2, dr]
+        cp = np.zeros(2 * G, COPY_DESC)
+        A_, B_ = cp[:G], cp[G:]
+        A_["dst"] = [self._ptr(j, b) for j, b in zip(jj, bb)]
+        B_["dst"] = [self._ptr(j, b + 1) for j, b in zip(jj, bb)]
+        # left: A' = V[:, :k] * S (col scale), B' = P * (1/S) (row scale)
+        A_["src"] = np.where(left, vptr, P.data_ptr() + 16 * poff)
+        A_["rows"], A_["cols"] = m, k
+        A_["s0"] = np.where(left, gld, k)
+        A_["s1"] = 1
+        A_["col_scale"] = np.where(left, sptr, 0)
+        B_["src"] = np.where(left, P.data_ptr() + 16 * poff, vptr)
+        B_["rows"], B_["cols"] = k, n
+        # right: B' = V_k^H : (r,c) = conj V[c, r]
+        B_["s0"] = np.where(left, n, 1)
+        B_["s1"] = np.where(left, 1, gld)
+        B_["conj"] = np.where(left, 0, 1)
+        B_["row_scale"] = np.where(left, iptr, 0)
+        cnt = cp["rows"] * cp["cols"]
+        pre, total = _prefix(cnt)
+        dd, pd = _upload(cp, self.dev), torch.from_numpy(pre).to(self.dev)
+
+        def runc(g):
+            check(lib().aq_grouped_copy(_vp(dd), _vp(pd), len(cp), total, g, _stream()),
+                  "aq_grouped_copy")
+        runc(self.tuner.grid("elem", total, None))
+        self.dims[jj, bb + 1] = k
+        self.logs["mps"].extend(jj.tolist())
+        self.logs["bond"].extend(bb.tolist())
+        self.logs["k"].extend(k.tolist())
+        self.logs["eps_local"].extend(rr["eps_local"].tolist())
+        self.logs["entropy"].extend(rr["entropy"].tolist())
+        # keep the temporaries alive until the stream has consumed them
+        self._keep = (X, X64, buckets, W_all, V_all, Sflat, Sf64, inv, P, dd, pd)
+
+    # ------------------------------------------------------------ circuits
+    def run_hea(self, circuits):
+        """circuits[j]: op list of atlas_q.circuits.hea (same layer structure for
+        all MPS: 2n 1q ops then CZ on even bonds then odd bonds per layer)."""
+        pos = [0] * self.M
+        while True:
+            any_left = False
+            one = [[] for _ in range(self.M)]
+            for j, ops in enumerate(circuits):
+                # a 1q block: consecutive 1q ops, merged per site (U^T convention:
+                # applying U1 then U2 equals applying U1 @ U2)
+                merged = {}
+                while pos[j] < len(ops) and ops[pos[j]][0] == "1q":
+                    _, q, U = ops[pos[j]]
+                    merged[q] = merged[q] @ U if q in merged else np.asarray(U)
+                    pos[j] += 1
+                one[j] = sorted(merged.items())
+                any_left |= bool(one[j])
+            self.apply_1q_layer(one)
+            two = [[] for _ in range(self.M)]
+            for j, ops in enumerate(circuits):
+                used = set()
+                while pos[j] < len(ops) and ops[pos[j]][0] == "2q":
+                    b = ops[pos[j]][1]
+                    if b in used or b - 1 in used or b + 1 in used:
+                        break
+                    used.add(b)
+                    two[j].append((b, ops[pos[j]][2]))
+                    pos[j] += 1
+                any_left |= bool(two[j])
+            self.apply_2q_layer(two)
+            if not any_left:
+                break
+
+    # ------------------------------------------------------------ sampling
+    def prepare_sampler(self, shots: int):
+        """Allocate sweep buffers and the per-site descriptor lists."""
+        S = int(shots)
+        self.shots = S
+        words = (self.n + 63) // 64
+        self.u_off, ut = _prefix(S * self.n)
+        self.uniforms = torch.empty(max(ut, 1), dtype=torch.float64, device=self.dev)
+        self.b_off, bt = _prefix(S * words)
+        self.bits = torch.empty(max(bt, 1), dtype=torch.int64, device=self.dev)
+        cmax = self.dims.max(axis=1)
+        tb = S * 2 * cmax
+        self.t_off, tt = _prefix(2 * tb)
+        self.tbuf = torch.empty(max(tt, 1), dtype=C64, device=self.dev)
+        pc = S * _cdiv(cmax, 32) * 2
+        self.p_off, pt = _prefix(pc)
+        self.part = torch.empty(max(pt, 1), dtype=torch.float32, device=self.dev)
+        self.o_off, ot = _prefix(np.full(self.M, S))
+        self.offs = torch.empty(max(ot, 1), dtype=torch.int32, device=self.dev)
+        self._site_lists = []
+        for i in range(self.nmax):
+            js = np.nonzero(self.n > i)[0]
+            d = np.zeros(len(js), SAMPLE_DESC)
+            for r, j in enumerate(js):
+                cl, cr = int(self.dims[j, i]), int(self.dims[j, i + 1])
+                cur = self.tbuf.data_ptr() + 8 * (int(self.t_off[j]) + (i & 1) * int(tb[j]))
+                prev = self.tbuf.data_ptr() + 8 * (int(self.t_off[j]) + ((i + 1) & 1) * int(tb[j]))
+                d[r]["L"] = 0 if i == 0 else prev
+                d[r]["off"] = 0 if i == 0 else self.offs.data_ptr() + 4 * int(self.o_off[j])
+                d[r]["site"] = self._ptr(j, i)
+                d[r]["Tout"] = cur
+                d[r]["part"] = self.part.data_ptr() + 4 * int(self.p_off[j])
+                d[r]["uniforms"] = self.uniforms.data_ptr() + 8 * int(self.u_off[j])
+                d[r]["bits"] = self.bits.data_ptr() + 8 * int(self.b_off[j])
+                d[r]["off_next"] = self.offs.data_ptr() + 4 * int(self.o_off[j])
+                d[r]["ldL"] = 2 * cl
+                d[r]["S"], d[r]["chiL"], d[r]["chiR"] = S, cl, cr
+                d[r]["n"], d[r]["site_index"], d[r]["words"] = self.n[j], i, words[j]
+            tiles = _cdiv(np.full(len(js), S), 64) * _cdiv(d["chiR"], 64)
+            tpre, ttot = _prefix(tiles)
+            spre, stot = _prefix(np.full(len(js), S))
+            self._site_lists.append((_upload(d, self.dev), torch.from_numpy(tpre).to(self.dev),
+                                     torch.from_numpy(spre).to(self.dev), len(js), ttot, stot))
+        self._grid_sample = self.tuner.grid("sample", self._site_lists[0][4], None)
+
+    def sweep(self):
+        """The full sweep for all MPS (launches only; uniforms must be filled)."""
+        check(lib().aq_zero_u64(_vp(self.bits), self.bits.numel(), _stream()), "aq_zero_u64")
+        for dd, tp, sp, cnt, ttot, stot in self._site_lists:
+            check(lib().aq_sample_grouped_site(_vp(dd), _vp(tp), _vp(sp), cnt, ttot, stot,
+                                               self._grid_sample, _stream()),
+                  "aq_sample_grouped_site")
+
+    def sample_bits(self, j: int) -> torch.Tensor:
+        words = (int(self.n[j]) + 63) // 64
+        o = int(self.b_off[j])
+        return self.bits[o: o + self.shots * words].view(self.shots, words)
+
+    def uniforms_of(self, j: int) -> torch.Tensor:
+        o = int(self.u_off[j])
+        return self.uniforms[o: o + self.shots * int(self.n[j])].view(self.shots, int(self.n[j]))

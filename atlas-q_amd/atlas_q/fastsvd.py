@@ -46,6 +46,19 @@ class FactorStats:
         self.counts[key] = self.counts.get(key, 0) + 1
 
 
+_DEFAULT = {}
+
+
+def default_predictor(device):
+    """The shipped trained rank predictor (rank_mlp_weights.npz), loaded once per device."""
+    key = str(device)
+    if key not in _DEFAULT:
+        from .rank_mlp import DEFAULT_WEIGHTS, RankPredictor
+        _DEFAULT[key] = RankPredictor.load(DEFAULT_WEIGHTS, device) if DEFAULT_WEIGHTS.exists() \
+            else None
+    return _DEFAULT[key]
+
+
 def _gram_factors(X: torch.Tensor):
     """Exact leading factors of X via the FP64 Gram matrix of the smaller side.
     Returns (U [m, r] c128, S [r] f64 descending, Vh [r, n] c128, X64)."""
@@ -111,6 +124,8 @@ def two_site_update(mps, i: int, A: torch.Tensor, B: torch.Tensor, gate: np.ndar
         return A_new, B_new, k, e, s, c, driver, local
     feats = None
     pred = getattr(mps, "rank_predictor", None)
+    if pred is None and method == "learned":
+        pred = mps.rank_predictor = default_predictor(X.device)
     if pred is not None and (method == "learned" or getattr(mps, "harvest", None) is not None):
         feats = pred.features(mps, i, X, gate, cap)
     out = None

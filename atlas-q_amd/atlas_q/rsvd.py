@@ -7,17 +7,23 @@ For X [m, n] (complex64) and a width l = k_guess + p:
   B = Q^H X        [l, n]          FP64 MFMA GEMM
   B B^H = U_B diag(lam) U_B^H      FP64 Hermitian core
   tail = ||X||_F^2 - sum(lam)      FP64: weight outside range(Q)
-The rank rule runs on [sqrt(lam), sqrt(tail)] (the cumulative energies of the
-true spectrum up to the captured part, with the exact total), so the
-reference's k decision is reproduced whenever the captured part resolves it.
-The result is accepted only when k < l and the tail weight is below
-TAIL_TOL * max(eps^2, 2^-24) * ||X||^2; otherwise the caller falls back to
-the exact Gram factorization.
+The result is accepted only when (i) the rank decision is certified: with
+captured energies E_i (descending, lower bounds of the true partial sums by
+Ky Fan) and the exact total ||X||_F^2, the first index k with E_{k-1} >=
+thresh + margin satisfies E_{k-2} + tail < thresh - margin (the true partial
+sums are at most E_i + tail), so neither the uncaptured weight nor the
+rule's rounding in the storage precision (margin = 1e-6 ||X||^2) can move k,
+and the reference's rounding of the rule (run on [sqrt(lam), sqrt(tail)])
+agrees;
+and (ii) the tail is at most TAIL_TOL * max(eps^2, 2^-24) * ||X||_F^2, which
+bounds the kept-weight (fidelity) difference from the exact truncation.
+Otherwise the caller falls back to the exact Gram factorization.
 """
 from __future__ import annotations
 
 import math
 
+import numpy as np
 import torch
 
 from . import kernels as K
@@ -25,9 +31,10 @@ from . import roofline as RL
 from .eig import eigh_desc
 from .truncation import choose_rank_from_sigma
 
-TAIL_TOL = 1e-3
-POWER_STEPS = 1
-WIDE_FRACTION = 0.75  # predicted widths beyond this share of min(m, n) go to the Gram path
+TAIL_TOL = 1e-2
+POWER_STEPS = 2
+WIDE_FRACTION = 0.75
+RULE_MARGIN = 1e-6     # of ||X||_F^2, around the rank-rule threshold  # predicted widths beyond this share of min(m, n) go to the Gram path
 
 
 def orth(Y: torch.Tensor) -> torch.Tensor:
@@ -49,7 +56,7 @@ def width_for(k_guess: int, m: int, n: int, cap: int):
     """Range-finder width for a rank guess: k_guess plus oversampling."""
     r = min(m, n)
     k_guess = max(1, min(k_guess, cap, r))
-    p = max(8, k_guess // 8)
+    p = max(10, k_guess // 4)
     return k_guess, min(r, k_guess + p)
 
 
@@ -91,6 +98,18 @@ def try_rsvd(mps, i, X, cap, budget_ok, local, cl, cr, st, k_guess):
     S_rule = S_aug.to(torch.float32) if local == torch.complex64 else S_aug
     k, eps_loc, entropy, condS = choose_rank_from_sigma(S_rule, eps, cap, budget_ok)
     ok = (k < l) and tail <= TAIL_TOL * max(eps * eps, 2.0 ** -24) * total
+    if ok:
+        # decide in FP64 with a margin that covers the rule's rounding in the
+        # storage precision (float32 energies round at ~3e-8 of the total)
+        E = torch.cumsum(lam, 0).cpu().numpy()
+        thresh = (1.0 - eps * eps) * total
+        margin = RULE_MARGIN * total
+        hit = np.nonzero(E >= thresh + margin)[0]
+        k_c = int(hit[0]) + 1 if hit.size else None
+        if k_c is None:
+            ok = k == cap and k < l and E[k - 1] + tail < thresh - margin
+        else:
+            ok = (k == min(k_c, cap)) and (k_c == 1 or E[k_c - 2] + tail < thresh - margin)
     st.predicted[-1] = (k_guess, k)
     if not ok:
         return None

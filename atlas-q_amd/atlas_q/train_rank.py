@@ -9,7 +9,8 @@ features of the theta matrix are recorded, after it the rank the reference's
 rule kept.  Circuit families: Haar brickwork, weakly entangling brickwork
 (exp(-i a H), H Hermitian Gaussian, a in [0.03, 0.6]) and the RY/RZ + CZ
 hardware-efficient ansatz, over chain lengths 8-40, caps 8-256 and cutoffs
-1e-2 .. 1e-8.  Split by circuit (90/10), as the reference's dataset builder
+1e-2 .. 1e-8; and steady-state slices at bond 32-256 with flat or decaying
+Schmidt profiles (``atlas_q.workloads``) under Haar or weak gates.  Split by circuit (90/10), as the reference's dataset builder
 splits by file (scripts/ai_tools/build_ft_dataset.py:103-108).
 """
 from __future__ import annotations
@@ -37,6 +38,21 @@ def weak_brickwork(n, depth, seed, angle):
     return ops
 
 
+def slice_job(rng):
+    """Steady-state slices (bonds chi at both ends) with flat or decaying
+    Schmidt profiles, Haar or weakly entangling gates."""
+    from .workloads import gate_schedule, slice_tensors
+    chi = int(2 ** rng.integers(5, 9))
+    kind = "flat" if rng.random() < 0.3 else "decay"
+    tau = float(rng.uniform(3.0, 40.0))
+    angle = None if rng.random() < 0.3 else float(rng.uniform(0.02, 0.6))
+    eps = float(10.0 ** -rng.integers(2, 9))
+    sites = int(rng.integers(4, 7))
+    ts = slice_tensors(sites, chi, kind, seed=int(rng.integers(1 << 30)), tau=tau)
+    gates = gate_schedule(sites, int(rng.integers(4, 10)), int(rng.integers(1 << 30)), angle)
+    return chi, eps, ts, gates
+
+
 def circuit(rng, cid):
     fam = cid % 3
     n = int(rng.integers(8, 41))
@@ -59,9 +75,17 @@ def harvest(pairs: int, seed: int = 0, max_seconds: float = 600.0, device="cuda"
     cid = 0
     t0 = time.time()
     while len(K) < pairs and time.time() - t0 < max_seconds:
-        n, cap, eps, ops = circuit(rng, cid)
-        m = AdaptiveMPS(n, bond_dim=1, eps_bond=eps, chi_max_per_bond=cap, device=device,
-                        svd_method="plain", rank_predictor=pred)
+        if cid % 4 == 3:
+            cap, eps, ts, gates = slice_job(rng)
+            m = AdaptiveMPS(len(ts), bond_dim=1, eps_bond=eps, chi_max_per_bond=cap,
+                            device=device, svd_method="plain", rank_predictor=pred)
+            m.tensors = ts
+            m.bond_dims = [cap] * (len(ts) - 1)
+            ops = [("2q", b, U) for b, U in gates]
+        else:
+            n, cap, eps, ops = circuit(rng, cid)
+            m = AdaptiveMPS(n, bond_dim=1, eps_bond=eps, chi_max_per_bond=cap, device=device,
+                            svd_method="plain", rank_predictor=pred)
         m.harvest = []
         for op in ops:
             U = torch.from_numpy(op[2])

@@ -22,8 +22,8 @@ void set_error(const char* fmt, ...) {
 // the cumulative sum is a single sequential double chain (the CPU cumsum's
 // accumulation order) run by lane 0; the entropy is a parallel reduction.
 template <typename T>
-__global__ __launch_bounds__(256) void rank_kernel(const T* __restrict__ S, int64_t m, T c,
-                                                   int64_t cap, aq_rank_result_t* out) {
+__device__ void rank_block(const T* __restrict__ S, int64_t m, T c, int64_t cap,
+                           aq_rank_result_t* out) {
   constexpr int CH = 2048;
   __shared__ T sw[CH];
   __shared__ double red[256];
@@ -141,6 +141,21 @@ __global__ __launch_bounds__(256) void rank_kernel(const T* __restrict__ S, int6
   }
 }
 
+template <typename T>
+__global__ __launch_bounds__(256) void rank_kernel(const T* __restrict__ S, int64_t m, T c,
+                                                   int64_t cap, aq_rank_result_t* out) {
+  rank_block<T>(S, m, c, cap, out);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void rank_batched_kernel(const void* const* __restrict__ Sp,
+                                                           const int64_t* __restrict__ m,
+                                                           const int64_t* __restrict__ cap, T c,
+                                                           aq_rank_result_t* out) {
+  const int b = blockIdx.x;
+  rank_block<T>((const T*)Sp[b], m[b], c, cap[b] < 1 ? 1 : cap[b], out + b);
+}
+
 // ---------------------------------------------------------------- rebuild
 // A'[r, c] = U[r, c] * S[c] (r < rows, c < k), B'[r, c] = Vh[r, c] (r < k,
 // c < cols); 32x32 tiles through LDS so both the read (either storage order
@@ -223,6 +238,16 @@ extern "C" int aq_version(void) { return 10000; }
 
 extern "C" const char* aq_last_error(void) { return g_err; }
 
+extern "C" int aq_desc_sizes(int64_t* out) {
+  AQ_REQUIRE(out, AQ_ERR_ARG, "aq_desc_sizes: null pointer");
+  out[0] = (int64_t)sizeof(aq_gemm_desc_t);
+  out[1] = (int64_t)sizeof(aq_site1q_desc_t);
+  out[2] = (int64_t)sizeof(aq_sample_desc_t);
+  out[3] = (int64_t)sizeof(aq_copy_desc_t);
+  out[4] = (int64_t)sizeof(aq_rank_result_t);
+  return AQ_OK;
+}
+
 extern "C" int aq_rank_select(int s_dtype, const void* S, int64_t m, double eps, int64_t cap,
                               int mode, aq_rank_result_t* out, hipStream_t stream) {
   AQ_REQUIRE(S && out, AQ_ERR_ARG, "aq_rank_select: null pointer");
@@ -269,6 +294,22 @@ extern "C" int aq_apply_1q(int dtype, const void* T_in, void* T_out, int64_t chi
     apply_1q_kernel<double><<<blocks, 256, 0, stream>>>(
         (const double2*)T_in, (double2*)T_out, chiL, chiR, mk<double>(g, 0), mk<double>(g, 1),
         mk<double>(g, 2), mk<double>(g, 3));
+  }
+  AQ_LAUNCH_CHECK();
+  return AQ_OK;
+}
+
+extern "C" int aq_rank_select_batched(int s_dtype, const void* const* S_ptrs, const int64_t* m,
+                                      const int64_t* cap, double eps, int count,
+                                      aq_rank_result_t* out, hipStream_t stream) {
+  AQ_REQUIRE(S_ptrs && m && cap && out, AQ_ERR_ARG, "aq_rank_select_batched: null pointer");
+  AQ_REQUIRE(count >= 1, AQ_ERR_ARG, "aq_rank_select_batched: count must be >= 1");
+  const double c = 1.0 - eps * eps;
+  if (s_dtype == AQ_C64) {
+    rank_batched_kernel<float><<<count, 256, 0, stream>>>(S_ptrs, m, cap, (float)c, out);
+  } else {
+    AQ_REQUIRE(s_dtype == AQ_C128, AQ_ERR_ARG, "aq_rank_select_batched: unknown dtype");
+    rank_batched_kernel<double><<<count, 256, 0, stream>>>(S_ptrs, m, cap, c, out);
   }
   AQ_LAUNCH_CHECK();
   return AQ_OK;

@@ -43,6 +43,9 @@ extern "C" {
 
 int aq_version(void);
 const char* aq_last_error(void);
+/* sizeof of aq_gemm_desc_t, aq_site1q_desc_t, aq_sample_desc_t, aq_copy_desc_t,
+ * aq_rank_result_t (5 int64), so bindings can check their struct layouts. */
+int aq_desc_sizes(int64_t* out);
 
 /* Theta: X = gate-applied two-site contraction, "U" convention
  *   X[2l+p, q*chiR+j] = sum_{p1,p2} U[2p+q, 2p1+p2] sum_m A[l,p1,m] B[m,p2,j].
@@ -159,6 +162,82 @@ int aq_mlp_backward(const float* X, const float* dy, int64_t batch, const uint16
 int aq_mlp_adamw(float* master, const float* grad, float* m, float* v, uint16_t* wb, double lr,
                  double beta1, double beta2, double eps, double wd, double grad_scale, int step,
                  hipStream_t stream);
+
+/* ---------------------------------------------------------------- grouped
+ * Batched, uneven work for many independent MPS in one persistent launch (no
+ * reference counterpart: the reference advances one MPS at a time).  The
+ * caller uploads one descriptor per problem and `prefix`, the exclusive prefix
+ * sum of the problems' work counts (tiles or elements, int64); a grid of
+ * `grid` work-groups strides over the global work index. */
+#define AQ_GROUPED_GEMM 0  /* C = op(A) op(B), linear columns */
+#define AQ_GROUPED_THETA 1 /* theta: A [chiL,2,chiM], B [chiM,2,chiR], C = X,
+                              M = 2chiL, N = 2chiR, K = chiM, gate in `gate` */
+typedef struct {
+  const void* A;
+  const void* B;
+  void* C;
+  int64_t M, N, K;
+  int64_t a_s0, a_s1, b_s0, b_s1, c_s0, c_s1; /* element strides */
+  int32_t conj_a, conj_b, kind, pad;
+  double gate[32]; /* AQ_GROUPED_THETA: U[4][4] interleaved re/im, row-major */
+} aq_gemm_desc_t;
+/* Tiles of a problem: ceil(M/64) * ceil(N/128) (GEMM) or ceil(M/64) *
+ * ceil(N/2/64) (theta).  layout: bit0 = A unit stride along k (a_s1 == 1),
+ * bit1 = B unit stride along n (b_s1 == 1), uniform over the group. */
+int aq_grouped_gemm(int dtype, const aq_gemm_desc_t* descs, const int64_t* prefix,
+                    int num_problems, int64_t total_tiles, int layout, int grid,
+                    hipStream_t stream);
+
+/* Single-qubit gates on many complex64 sites (AdaptiveMPS convention,
+ * adaptive_mps.py:177): work count of a site = chiL * chiR. */
+typedef struct {
+  const void* T_in;
+  void* T_out;
+  int64_t chiL, chiR;
+  double gate[8];
+} aq_site1q_desc_t;
+int aq_grouped_1q(const aq_site1q_desc_t* descs, const int64_t* prefix, int num_sites,
+                  int64_t total_elems, int grid, hipStream_t stream);
+
+/* Rank rule (aq_rank_select semantics) on `count` spectra at once, one
+ * work-group each: S_ptrs / m / cap are device arrays of length count. */
+int aq_rank_select_batched(int s_dtype, const void* const* S_ptrs, const int64_t* m,
+                           const int64_t* cap, double eps, int count, aq_rank_result_t* out,
+                           hipStream_t stream);
+
+/* Sweep sampler over many MPS: one grouped GEMM launch and one grouped select
+ * launch per site index (aq_sample semantics per MPS).  For site index i the
+ * caller passes the descriptors of every MPS with more than i sites. */
+typedef struct {
+  const void* L;       /* previous site's T rows (NULL at site 0) */
+  const int32_t* off;  /* per-shot column offset into L (NULL at site 0) */
+  const void* site;    /* [chiL, 2, chiR] */
+  void* Tout;          /* [S, 2 chiR] */
+  void* part;          /* [S, ceil(chiR/32), 2] real */
+  const double* uniforms; /* [S, n] */
+  uint64_t* bits;      /* [S, words] */
+  int32_t* off_next;   /* [S] */
+  int64_t ldL, S, chiL, chiR, n, site_index, words;
+} aq_sample_desc_t;
+int aq_sample_grouped_site(const aq_sample_desc_t* descs, const int64_t* tile_prefix,
+                           const int64_t* shot_prefix, int count, int64_t total_tiles,
+                           int64_t total_shots, int grid, hipStream_t stream);
+int aq_zero_u64(uint64_t* p, int64_t count, hipStream_t stream);
+
+/* Grouped core rebuild (adaptive_mps.py:280-290 for many updates): complex64
+ * dst[r, c] = op(src[r*s0 + c*s1]) * row_scale[r] * col_scale[c] for r < rows,
+ * c < cols, src complex128 (FP64 factors), scales float64 (nullable); work
+ * count of a descriptor = rows * cols. */
+typedef struct {
+  const void* src;
+  void* dst;
+  const double* row_scale;
+  const double* col_scale;
+  int64_t rows, cols, s0, s1;
+  int32_t conj, pad;
+} aq_copy_desc_t;
+int aq_grouped_copy(const aq_copy_desc_t* descs, const int64_t* prefix, int count,
+                    int64_t total_elems, int grid, hipStream_t stream);
 
 #ifdef __cplusplus
 }

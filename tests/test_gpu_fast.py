@@ -99,31 +99,63 @@ class _ReplayPredictor:
         return k
 
 
+def _slice_run(method, tensors, gates, chi, eps, **kw):
+    m = AdaptiveMPS(len(tensors), bond_dim=1, eps_bond=eps, chi_max_per_bond=chi, device=DEV,
+                    svd_method=method, **kw)
+    m.tensors = [t.clone() for t in tensors]
+    m.bond_dims = [t.shape[2] for t in tensors[:-1]]
+    for b, U in gates:
+        m.apply_two_site_gate(b, torch.from_numpy(U))
+    return m
+
+
+def _slice_fidelity(a, b):
+    """|<a|b>|^2 / (<a|a><b|b>) for two slices with open end bonds (the ends
+    traced pairwise: each slice is read as an operator between its end bonds)."""
+    def transfer(x_list, y_list):
+        E = torch.eye(x_list[0].shape[0], dtype=torch.complex128)
+        for x, y in zip(x_list, y_list):
+            x = x.cpu().to(torch.complex128)
+            y = y.cpu().to(torch.complex128)
+            E = torch.einsum("xy,xsa,ysb->ab", E, x.conj(), y)
+        return complex(torch.trace(E))
+    ov = transfer(a.tensors, b.tensors)
+    return abs(ov) ** 2 / (transfer(a.tensors, a.tensors).real * transfer(b.tensors, b.tensors).real)
+
+
+def _first_layer(seed, chi=256, sites=9, tau=10.0, angle=0.1):
+    """A fresh low-entanglement slice (all bonds chi, decaying Schmidt
+    profile) and one brickwork layer of weak gates on its even bonds: each
+    update contracts a 2chi x 2chi theta whose kept rank is far below chi."""
+    from atlas_q.workloads import gate_schedule, slice_tensors
+    ts = slice_tensors(sites, chi, "decay", seed=seed, tau=tau)
+    gates = gate_schedule(sites, (sites - 1 + 1) // 2, seed=seed + 1, angle=angle)
+    return ts, gates
+
+
 def test_rsvd_certified_results_match_gram():
-    # cutoff 1e-2 is not float32-saturated: the kept rank is well below 2 chi
-    gates = _weak_gates(16, 12, 4)
-    gram = _run("gram", n=16, gates=gates, chi=64, eps=1e-2)
-    ks = gram.statistics.logs["k_star"]
-    rs = _run("learned", n=16, gates=gates, chi=64, eps=1e-2,
-              rank_predictor=_ReplayPredictor([int(1.3 * k) + 2 for k in ks]))
-    c = rs.factor_stats.counts
-    assert c["rsvd"] > len(ks) // 2, c
-    assert rs.statistics.logs["k_star"] == ks
-    np.testing.assert_allclose(rs.statistics.logs["eps_local"],
-                               gram.statistics.logs["eps_local"], rtol=1e-3, atol=1e-6)
-    a = gram.to_statevector().cpu().numpy().astype(np.complex128)
-    b = rs.to_statevector().cpu().numpy().astype(np.complex128)
-    fid = abs(np.vdot(a, b)) ** 2 / (np.vdot(a, a).real * np.vdot(b, b).real)
-    assert abs(1 - fid) < 1e-6
+    n_rsvd, n_total = 0, 0
+    for seed in (1, 2, 3):
+        ts, gates = _first_layer(seed)
+        gram = _slice_run("gram", ts, gates, 256, 1e-3)
+        ks = gram.statistics.logs["k_star"]
+        rs = _slice_run("learned", ts, gates, 256, 1e-3,
+                        rank_predictor=_ReplayPredictor([int(1.2 * k) + 2 for k in ks]))
+        n_rsvd += rs.factor_stats.counts["rsvd"]
+        n_total += len(ks)
+        assert rs.statistics.logs["k_star"] == ks
+        np.testing.assert_allclose(rs.statistics.logs["eps_local"],
+                                   gram.statistics.logs["eps_local"], rtol=1e-2, atol=1e-6)
+        assert abs(1 - _slice_fidelity(gram, rs)) < 1e-6
+    assert n_rsvd >= n_total // 2, (n_rsvd, n_total)
 
 
 def test_rsvd_under_prediction_falls_back():
-    gates = _weak_gates(12, 8, 5)
-    gram = _run("gram", n=12, gates=gates, chi=64, eps=1e-2)
+    ts, gates = _first_layer(7)
+    gram = _slice_run("gram", ts, gates, 256, 1e-3)
     ks = gram.statistics.logs["k_star"]
-    low = [max(1, k // 3) for k in ks]
-    rs = _run("learned", n=12, gates=gates, chi=64, eps=1e-2,
-              rank_predictor=_ReplayPredictor(low))
+    rs = _slice_run("learned", ts, gates, 256, 1e-3,
+                    rank_predictor=_ReplayPredictor([max(1, k // 4) for k in ks]))
     assert rs.factor_stats.counts["rsvd_fallback"] > 0
     assert rs.statistics.logs["k_star"] == ks
 
