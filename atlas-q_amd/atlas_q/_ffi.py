@@ -90,7 +90,8 @@ _lib = None
 def header_symbols() -> list:
     """Entry points declared in include/atlasq.h."""
     txt = HEADER.read_text()
-    return sorted(set(re.findall(r"^\s*(?:const\s+)?[\w\*\s]+?\b(aq_\w+)\s*\(", txt, re.M)))
+    decl = r"^(?:const\s+)?(?:int|int64_t|char|void)\s*\**\s*(aq_\w+)\s*\("
+    return sorted(set(re.findall(decl, txt, re.M)))
 
 
 def lib() -> ctypes.CDLL:
