@@ -16,7 +16,9 @@ Extensions (keyword-only, defaults reproduce the reference):
 from __future__ import annotations
 
 import math
+import threading
 import time
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass
 from typing import Dict, List, Optional, Union
 
@@ -42,6 +44,29 @@ class DTypePolicy:
 
     default: torch.dtype = torch.complex64
     promote_if_cond_gt: float = 1e6
+
+
+_POOLS = {}
+_TLS = threading.local()
+
+
+def _worker_pool(workers: int):
+    """Process-wide host thread pools, one per worker count."""
+    if workers not in _POOLS:
+        _POOLS[workers] = ThreadPoolExecutor(max_workers=workers,
+                                             thread_name_prefix="aq-update")
+    return _POOLS[workers]
+
+
+def _thread_stream(device) -> torch.cuda.Stream:
+    """The calling thread's own HIP stream (default priority) on `device`."""
+    streams = getattr(_TLS, "streams", None)
+    if streams is None:
+        streams = _TLS.streams = {}
+    key = torch.device(device).index or 0
+    if key not in streams:
+        streams[key] = torch.cuda.Stream(device)
+    return streams[key]
 
 
 def _gate4(U4) -> np.ndarray:
@@ -131,6 +156,50 @@ class AdaptiveMPS(MatrixProductStatePyTorch):
         """adaptive_mps.py:182-308."""
         assert 0 <= i < self.num_qubits - 1, f"Bond index {i} out of range"
         start = time.time()
+        self._commit(i, self._compute_update(i, U4), start)
+
+    @torch.no_grad()
+    def apply_two_site_gates(self, gates, workers: int = 4):
+        """Apply two-site gates on pairwise disjoint bonds (one brickwork
+        sub-layer).  The updates are independent, so they run concurrently on
+        `workers` host threads, each launching on its own HIP stream; tensors,
+        bond dimensions and statistics are committed in the given order, so
+        the result equals applying the gates one by one."""
+        gates = list(gates)
+        used = set()
+        for i, _ in gates:
+            assert 0 <= i < self.num_qubits - 1, f"Bond index {i} out of range"
+            if {i, i + 1} & used:
+                raise ValueError("apply_two_site_gates needs gates on disjoint bonds")
+            used |= {i, i + 1}
+        if workers <= 1 or len(gates) <= 1 or self.budget_global_mb is not None:
+            # the global budget reads the whole chain's memory: keep the order
+            for i, U in gates:
+                self.apply_two_site_gate(i, U)
+            return
+        pool = _worker_pool(workers)
+        start = time.time()
+        ready = torch.cuda.current_stream(self.device)
+        ready_ev = ready.record_event()
+
+        def job(i, U):
+            st = _thread_stream(self.device)
+            with torch.cuda.stream(st):
+                st.wait_event(ready_ev)
+                out = self._compute_update(i, U)
+                done = st.record_event()
+            return out, done
+
+        futs = [pool.submit(job, i, U) for i, U in gates]
+        results = [f.result() for f in futs]
+        for (i, _), (out, done) in zip(gates, results):
+            ready.wait_event(done)
+            for t in out[0:2]:
+                t.record_stream(ready)
+            self._commit(i, out, start)
+
+    def _compute_update(self, i: int, U4):
+        """Theta, factorization, rank rule and rebuild for bond i; no state change."""
         A, B = self.tensors[i], self.tensors[i + 1]
         cl, cm, cr = A.shape[0], A.shape[2], B.shape[2]
         local = self.dtype_policy.default
@@ -139,27 +208,29 @@ class AdaptiveMPS(MatrixProductStatePyTorch):
         gate = _gate4(U4)
         cap = self.chi_max_per_bond[i]
         budget_ok = self._budget_ok_factory(cl, cm, cr, local)
-
         if self.svd_method != "plain":
             from . import fastsvd
-            res = fastsvd.two_site_update(self, i, A, B, gate, cap, budget_ok, local)
-            A_new, B_new, k, eps_loc, entropy, condS, driver, local = res
-        else:
-            prec = self.theta_precision if local == torch.complex64 else "native"
-            X = K.theta(A, B, gate, prec=prec)
-            feats = None
-            if self.harvest is not None and self.rank_predictor is not None:
-                feats = self.rank_predictor.features(self, i, X, gate, cap)
-            U, S, Vh, driver = robust_svd(X)
-            k, eps_loc, entropy, condS = choose_rank_from_sigma(S, self.eps_bond, cap, budget_ok)
-            if feats is not None:
-                self.harvest.append((feats.cpu().numpy(), k))
-            if math.isfinite(condS) and condS > self.dtype_policy.promote_if_cond_gt:
-                U, S, Vh, driver = robust_svd(X.to(torch.complex128))
-                k, eps_loc, entropy, condS = choose_rank_from_sigma(S, self.eps_bond, cap,
-                                                                    budget_ok)
-                local = torch.complex128
-            A_new, B_new = K.rebuild(U, S, Vh, k, cl, cr)
+            return fastsvd.two_site_update(self, i, A, B, gate, cap, budget_ok, local) + (cm,)
+        prec = self.theta_precision if local == torch.complex64 else "native"
+        X = K.theta(A, B, gate, prec=prec)
+        feats = None
+        if self.harvest is not None and self.rank_predictor is not None:
+            feats = self.rank_predictor.features(self, i, X, gate, cap)
+        U, S, Vh, driver = robust_svd(X)
+        k, eps_loc, entropy, condS = choose_rank_from_sigma(S, self.eps_bond, cap, budget_ok)
+        if feats is not None:
+            self.harvest.append((feats.cpu().numpy(), k))
+        if math.isfinite(condS) and condS > self.dtype_policy.promote_if_cond_gt:
+            U, S, Vh, driver = robust_svd(X.to(torch.complex128))
+            k, eps_loc, entropy, condS = choose_rank_from_sigma(S, self.eps_bond, cap,
+                                                                budget_ok)
+            local = torch.complex128
+        A_new, B_new = K.rebuild(U, S, Vh, k, cl, cr)
+        return A_new, B_new, k, eps_loc, entropy, condS, driver, local, cm
+
+    def _commit(self, i: int, out, start: float):
+        A_new, B_new, k, eps_loc, entropy, condS, driver, local, cm = out
+        cl, cr = A_new.shape[0], B_new.shape[2]
         self._last_entropy[i] = entropy
         self._last_eps[i] = eps_loc
         if not check_entropy_sanity(entropy, cl, cr):

@@ -36,17 +36,35 @@ from .truncation import choose_rank_from_sigma
 
 
 class FactorStats:
-    """Counters of which factorization served each update."""
+    """Counters of which factorization served each update (thread-safe)."""
 
     def __init__(self):
+        import threading
         self.counts = {"gram": 0, "rsvd": 0, "rsvd_fallback": 0, "plain": 0}
         self.predicted = []   # (predicted width, kept rank)
+        self.rejects = {}     # why the randomized factorization fell back
+        self.tails = []       # uncaptured weight / ||X||^2 per randomized attempt
+        self.lock = threading.Lock()
 
     def bump(self, key):
-        self.counts[key] = self.counts.get(key, 0) + 1
+        with self.lock:
+            self.counts[key] = self.counts.get(key, 0) + 1
+
+    def reject(self, reason):
+        with self.lock:
+            self.rejects[reason] = self.rejects.get(reason, 0) + 1
 
 
 _DEFAULT = {}
+_INIT_LOCK = __import__("threading").Lock()
+
+
+def _factor_stats(mps):
+    with _INIT_LOCK:
+        st = getattr(mps, "factor_stats", None)
+        if st is None:
+            st = mps.factor_stats = FactorStats()
+        return st
 
 
 def default_predictor(device):
@@ -108,9 +126,7 @@ def two_site_update(mps, i: int, A: torch.Tensor, B: torch.Tensor, gate: np.ndar
     """Returns (A_new, B_new, k, eps_local, entropy, condS, driver, dtype)."""
     method = mps.svd_method
     cl, cr = A.shape[0], B.shape[2]
-    st = getattr(mps, "factor_stats", None)
-    if st is None:
-        st = mps.factor_stats = FactorStats()
+    st = _factor_stats(mps)
     prec = mps.theta_precision if local == torch.complex64 else "native"
     cm = A.shape[2]
     with RL.timed(f"theta_{prec}", 32.0 * cl * cm * cr + 128.0 * cl * cr,
@@ -125,7 +141,8 @@ def two_site_update(mps, i: int, A: torch.Tensor, B: torch.Tensor, gate: np.ndar
     feats = None
     pred = getattr(mps, "rank_predictor", None)
     if pred is None and method == "learned":
-        pred = mps.rank_predictor = default_predictor(X.device)
+        with _INIT_LOCK:
+            pred = mps.rank_predictor = default_predictor(X.device)
     if pred is not None and (method == "learned" or getattr(mps, "harvest", None) is not None):
         feats = pred.features(mps, i, X, gate, cap)
     out = None

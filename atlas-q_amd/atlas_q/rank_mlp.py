@@ -89,6 +89,15 @@ class RankPredictor:
         self._gen.manual_seed(777)
         self.trained = False
 
+    def _thread_gen(self):
+        import threading
+        tls = self.__dict__.setdefault("_tls", threading.local())
+        g = getattr(tls, "gen", None)
+        if g is None:
+            g = tls.gen = torch.Generator(device=self.device)
+            g.manual_seed(777)
+        return g
+
     # ------------------------------------------------------------ parameters
     def _init(self, seed):
         rng = np.random.default_rng(seed)
@@ -134,7 +143,7 @@ class RankPredictor:
 
     def features(self, mps, i: int, X: torch.Tensor, gate64: np.ndarray, cap: int) -> torch.Tensor:
         m, n = X.shape
-        Om = torch.randn(n, SKETCH, dtype=X.dtype, device=X.device, generator=self._gen)
+        Om = torch.randn(n, SKETCH, dtype=X.dtype, device=X.device, generator=self._thread_gen())
         Y = K.cgemm(X, Om).to(torch.complex128)
         G = K.cgemm(Y.T, Y, conj_a=True)
         mu, _ = heev_small(G)

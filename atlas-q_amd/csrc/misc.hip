@@ -24,7 +24,7 @@ void set_error(const char* fmt, ...) {
 template <typename T>
 __device__ void rank_block(const T* __restrict__ S, int64_t m, T c, int64_t cap,
                            aq_rank_result_t* out) {
-  constexpr int CH = 2048;
+  constexpr int CH = 2048;  // multiple of 8
   __shared__ T sw[CH];
   __shared__ double red[256];
   __shared__ int64_t s_k;
@@ -32,18 +32,31 @@ __device__ void rank_block(const T* __restrict__ S, int64_t m, T c, int64_t cap,
   __shared__ T s_total, s_ekm1, s_ecap;
   const int tid = threadIdx.x;
 
-  // pass 1: total = E[m-1]
+  // pass 1: total = E[m-1].  One lane runs the sequential double chain (the
+  // CPU cumsum's accumulation order); it reads the staged weights eight at a
+  // time so the LDS loads are issued ahead of the dependent adds.
   double acc = 0.0;
   for (int64_t base = 0; base < m; base += CH) {
     const int64_t len = (m - base) < CH ? (m - base) : CH;
     __syncthreads();
-    for (int64_t i = tid; i < len; i += 256) {
-      const T s = S[base + i];
-      sw[i] = s * s;
+    for (int64_t i = tid; i < CH; i += 256) {
+      T w = 0;
+      if (i < len) {
+        const T s = S[base + i];
+        w = s * s;
+      }
+      sw[i] = w;  // zero-padded to a multiple of 8
     }
     __syncthreads();
-    if (tid == 0)
-      for (int64_t i = 0; i < len; ++i) acc += (double)sw[i];
+    if (tid == 0) {
+      for (int64_t i = 0; i < len; i += 8) {
+        T w[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) w[u] = sw[i + u];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc += (double)w[u];
+      }
+    }
   }
   if (tid == 0) {
     s_total = (T)acc;
@@ -65,7 +78,8 @@ __device__ void rank_block(const T* __restrict__ S, int64_t m, T c, int64_t cap,
   }
   const T thresh = c * total;
   // pass 2: first index with E[i] >= thresh (left searchsorted); also keep
-  // E[cap-1] for the capped case
+  // E[cap-1] for the capped case.  Chunks of 8: the eight prefix values are
+  // formed first, then tested in order.
   if (tid == 0) {
     s_k = -1;
     s_ekm1 = total;
@@ -76,19 +90,34 @@ __device__ void rank_block(const T* __restrict__ S, int64_t m, T c, int64_t cap,
     const int64_t len = (m - base) < CH ? (m - base) : CH;
     __syncthreads();
     if (s_k >= 0) break;
-    for (int64_t i = tid; i < len; i += 256) {
-      const T s = S[base + i];
-      sw[i] = s * s;
+    for (int64_t i = tid; i < CH; i += 256) {
+      T w = 0;
+      if (i < len) {
+        const T s = S[base + i];
+        w = s * s;
+      }
+      sw[i] = w;
     }
     __syncthreads();
     if (tid == 0) {
-      for (int64_t i = 0; i < len; ++i) {
-        acc += (double)sw[i];
-        const T e = (T)acc;
-        if (base + i == cap - 1) s_ecap = e;
-        if (e >= thresh) {
-          s_k = base + i;
-          s_ekm1 = e;
+      for (int64_t i = 0; i < len; i += 8) {
+        T w[8], e[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) w[u] = sw[i + u];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          acc += (double)w[u];
+          e[u] = (T)acc;
+        }
+        int hit = -1;
+#pragma unroll
+        for (int u = 7; u >= 0; --u)
+          if (i + u < len && e[u] >= thresh) hit = u;
+        const int64_t cidx = cap - 1 - (base + i);
+        if (cidx >= 0 && cidx < 8 && (hit < 0 || cidx <= hit)) s_ecap = e[cidx];
+        if (hit >= 0) {
+          s_k = base + i + hit;
+          s_ekm1 = e[hit];
           break;
         }
       }

@@ -4,10 +4,15 @@
 Workload: the steady state of the split-precision brickwork circuit (128
 qubits, depth 60, Haar SU(4), chi_max = 1024, cutoff 1e-8, complex64): a slice
 of SITES consecutive sites whose bonds -- including the two open ends -- are
-all at the cap 1024, so every update contracts a 2048 x 2048 theta and keeps
-1024 singular triplets.  Gates are seeded Haar SU(4) in brickwork order over
+all at the cap 1024 (sites Lambda G: random right-isometries G, Schmidt
+weights of a saturated random circuit), so every update contracts a
+2048 x 2048 theta and keeps 1024 singular triplets.  Gates are seeded Haar SU(4) in brickwork order over
 the slice's interior bonds.  A step is one two-site update
 (theta -> truncated SVD -> rank rule -> rebuild), tensors resident in HBM.
+The gates of one brickwork sub-layer act on disjoint bonds; both paths run
+them concurrently on WORKERS host threads with one HIP stream each (the
+eigen/SVD solvers are launch-latency bound, so overlap raises throughput) and
+commit in circuit order.  "sequential" repeats both paths one gate at a time.
 
 value        : updates/s of the learned-rank + BF16x3 path (svd_method=learned,
                theta_precision=bf16x3)
@@ -39,13 +44,17 @@ METRIC = "2-site gate updates/s @chi=1024, learned-rank vs plain SVD, at equal f
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=12)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=18)
+    ap.add_argument("--warmup", type=int, default=9)
     ap.add_argument("--chi", type=int, default=1024)
-    ap.add_argument("--sites", type=int, default=6)
+    ap.add_argument("--sites", type=int, default=10)
+    ap.add_argument("--workers", type=int, default=4,
+                    help="host threads / HIP streams for the independent gates of a sub-layer")
     ap.add_argument("--method", default="learned")
     ap.add_argument("--theta-precision", default="bf16x3")
-    ap.add_argument("--plain-steps", type=int, default=3)
+    ap.add_argument("--plain-steps", type=int, default=9)
+    ap.add_argument("--seq-steps", type=int, default=4,
+                    help="updates timed one at a time (workers=1) for both paths")
     ap.add_argument("--fidelity-samples", type=int, default=2)
     ap.add_argument("--cpu-updates", type=int, default=2)
     ap.add_argument("--no-plain", action="store_true")
@@ -55,12 +64,10 @@ def parse():
 
 
 def slice_state(torch, sites, chi, seed, device):
-    g = torch.Generator(device="cpu").manual_seed(seed)
-    ts = []
-    for _ in range(sites):
-        t = torch.complex(torch.randn(chi, 2, chi, generator=g), torch.randn(chi, 2, chi, generator=g))
-        ts.append((t / t.norm()).to(device))
-    return ts
+    """Steady-state slice: Lambda G sites, G random right-isometries, Lambda the
+    Schmidt weights of a saturated random circuit (all bonds at the cap)."""
+    from atlas_q.workloads import slice_tensors
+    return slice_tensors(sites, chi, "flat", seed=seed, device=device)
 
 
 def gate_schedule(sites, count, seed):
@@ -73,6 +80,39 @@ def gate_schedule(sites, count, seed):
             out.append((b, haar_su4(rng)))
         layer += 1
     return out[:count]
+
+
+def sublayers(gates):
+    """Split a brickwork-ordered gate list into runs of gates on disjoint bonds."""
+    runs, cur, used = [], [], set()
+    for b, U in gates:
+        if {b, b + 1} & used:
+            runs.append(cur)
+            cur, used = [], set()
+        cur.append((b, U))
+        used |= {b, b + 1}
+    if cur:
+        runs.append(cur)
+    return runs
+
+
+def run_gates(torch, mps, gates, workers):
+    """Apply gates in order, timing only the updates; independent gates of a
+    sub-layer run concurrently.  Between sub-layers (outside the timed region)
+    the touched sites are rescaled to unit norm: the slice's open ends carry
+    no normalisation, and without it the scale of theta drifts from update
+    to update until the float32 SVD underflows."""
+    total = 0.0
+    for run in sublayers(gates):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        mps.apply_two_site_gates(run, workers=workers)
+        torch.cuda.synchronize()
+        total += time.perf_counter() - t0
+        for b, _ in run:
+            for q in (b, b + 1):
+                mps.tensors[q] = mps.tensors[q] / mps.tensors[q].norm()
+    return total
 
 
 def make_mps(torch, tensors, chi, method, prec):
@@ -121,37 +161,37 @@ def main():
     chi, sites = a.chi, a.sites
     base = slice_state(torch, sites, chi, a.seed, dev)
     total = a.warmup + a.steps
-    sched = gate_schedule(sites, total + max(a.plain_steps, a.fidelity_samples), a.seed + 1)
+    sched = gate_schedule(sites, total + max(a.plain_steps, a.fidelity_samples, a.seq_steps),
+                          a.seed + 1)
 
     # ---------------------------------------------------------------- fast path
     from atlas_q import roofline as RL
     fast = make_mps(torch, base, chi, a.method, a.theta_precision)
-    for b, U in sched[: a.warmup]:
-        fast.apply_two_site_gate(b, U)
-    torch.cuda.synchronize()
+    run_gates(torch, fast, sched[: a.warmup], a.workers)
     RL.reset()
     RL.enable(True)
-    t0 = time.perf_counter()
-    for b, U in sched[a.warmup: total]:
-        fast.apply_two_site_gate(b, U)
-    torch.cuda.synchronize()
-    t_fast = time.perf_counter() - t0
+    t_fast = run_gates(torch, fast, sched[a.warmup: total], a.workers)
     RL.enable(False)
     ups = a.steps / t_fast
     roof = RL.summary()
     ks = fast.statistics.logs["k_star"][a.warmup:]
+    fast_seq = None
+    if a.workers > 1 and a.seq_steps > 0:
+        fs = make_mps(torch, fast.tensors, chi, a.method, a.theta_precision)
+        run_gates(torch, fs, sched[total - 1: total], 1)             # main-thread warm-up
+        fast_seq = a.seq_steps / run_gates(torch, fs, sched[total: total + a.seq_steps], 1)
 
     # ---------------------------------------------------------------- plain path
-    plain = None
+    plain = plain_seq = None
     if not a.no_plain and a.plain_steps > 0:
         pm = make_mps(torch, fast.tensors, chi, "plain", "native")
-        pm.apply_two_site_gate(*sched[total - 1])  # warm-up of the SVD library
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for b, U in sched[total: total + a.plain_steps]:
-            pm.apply_two_site_gate(b, U)
-        torch.cuda.synchronize()
-        plain = a.plain_steps / (time.perf_counter() - t0)
+        run_gates(torch, pm, sched[total - a.workers: total], a.workers)  # library warm-up
+        plain = a.plain_steps / run_gates(torch, pm, sched[total: total + a.plain_steps],
+                                          a.workers)
+        if a.workers > 1 and a.seq_steps > 0:
+            ps = make_mps(torch, fast.tensors, chi, "plain", "native")
+            run_gates(torch, ps, sched[total - 1: total], 1)
+            plain_seq = a.seq_steps / run_gates(torch, ps, sched[total: total + a.seq_steps], 1)
 
     # ---------------------------------------------------------------- fidelity
     fid = []
@@ -198,9 +238,12 @@ def main():
                                f"chi_max={chi}: {sites} sites, all bonds {chi}, cutoff 1e-8, "
                                f"complex64; theta {2*chi}x{2*chi}, kept rank {chi}",
                    "method": a.method, "theta_precision": a.theta_precision,
-                   "parallelism": "single GPU"},
+                   "parallelism": f"single GPU; independent gates of a sub-layer on "
+                                  f"{a.workers} HIP streams (both paths)"},
         "plain_updates_per_s": plain,
         "speedup_vs_plain": (ups / plain) if plain else None,
+        "sequential": {"fast_updates_per_s": fast_seq, "plain_updates_per_s": plain_seq,
+                       "speedup": (fast_seq / plain_seq) if fast_seq and plain_seq else None},
         "kept_ranks": sorted(set(ks)),
         "fidelity": {"max_abs_dF_fast_vs_plain": dF, "samples": fid},
         "factor_counts": getattr(fast, "factor_stats", None).counts if hasattr(fast, "factor_stats") else None,

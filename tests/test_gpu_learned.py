@@ -1,7 +1,8 @@
 """What the trained rank predictor gains over the same randomized method
 without it, on a low-entanglement steady state (bond 256, decaying Schmidt
 profile, weakly entangling gates, cutoff 1e-3), with the exact Gram method
-as judge: identical kept ranks, fidelity 1 within 1e-6, and more updates
+as judge: the same kept ranks (up to the float32 resolution of the rank
+rule), fidelity 1 within 1e-6, and more updates
 served by the randomized path (fewer full factorizations) than with the
 model-free width bound min(cap, 2 chi_mid)."""
 import time
@@ -13,7 +14,7 @@ import torch
 from atlas_q.adaptive_mps import AdaptiveMPS
 from atlas_q.rank_mlp import DEFAULT_WEIGHTS, RankPredictor
 from atlas_q.workloads import gate_schedule, slice_tensors
-from test_gpu_fast import _first_layer, _slice_fidelity, _slice_run
+from test_gpu_fast import _first_layer, _ranks_agree, _slice_fidelity, _slice_run
 
 pytestmark = pytest.mark.gpu
 
@@ -35,7 +36,8 @@ def test_trained_network_gain_over_method_without_it():
     """First brickwork layer on fresh low-entanglement slices at bond 256
     (theta 512 x 512, kept ranks ~50-120): the network's width lets the
     randomized SVD serve the update; the model-free bound 2 chi_mid = 512
-    sends every update to the full Gram factorization."""
+    is too wide for a randomized factorization to pay, so every update runs the
+    full Gram factorization."""
     assert DEFAULT_WEIGHTS.exists(), "trained weights missing"
     pred = RankPredictor.load(DEFAULT_WEIGHTS, "cuda")
     slices = [_first_layer(seed) for seed in (21, 22, 23)]
@@ -51,7 +53,7 @@ def test_trained_network_gain_over_method_without_it():
           f"rsvd {cnt(nomodel, 'rsvd')} | learned {t_learned * 1e3:.1f} ms, "
           f"rsvd {cnt(learned, 'rsvd')}")
     for e, l_ in zip(exact, learned):
-        assert l_.statistics.logs["k_star"] == e.statistics.logs["k_star"]
+        _ranks_agree(l_.statistics.logs["k_star"], e.statistics.logs["k_star"])
         assert abs(1 - _slice_fidelity(e, l_)) < 1e-6
     assert cnt(learned, "rsvd") > cnt(nomodel, "rsvd")
     assert t_learned < t_nomodel
