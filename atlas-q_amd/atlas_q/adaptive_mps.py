@@ -161,10 +161,12 @@ class AdaptiveMPS(MatrixProductStatePyTorch):
     @torch.no_grad()
     def apply_two_site_gates(self, gates, workers: int = 4):
         """Apply two-site gates on pairwise disjoint bonds (one brickwork
-        sub-layer).  The updates are independent, so they run concurrently on
-        `workers` host threads, each launching on its own HIP stream; tensors,
-        bond dimensions and statistics are committed in the given order, so
-        the result equals applying the gates one by one."""
+        sub-layer).  The updates are independent: the truncated-SVD methods
+        factor all their Gram cores in one batched eigensolver call; the plain
+        method runs them concurrently on `workers` host threads, each
+        launching on its own HIP stream.  Tensors, bond dimensions and
+        statistics are committed in the given order, so the result equals
+        applying the gates one by one."""
         gates = list(gates)
         used = set()
         for i, _ in gates:
@@ -176,6 +178,13 @@ class AdaptiveMPS(MatrixProductStatePyTorch):
             # the global budget reads the whole chain's memory: keep the order
             for i, U in gates:
                 self.apply_two_site_gate(i, U)
+            return
+        if self.svd_method != "plain":
+            # truncated-SVD methods: one pass over the sub-layer on the current
+            # stream, the Gram cores of all its bonds factored in one batch
+            start = time.time()
+            for (i, _), out in zip(gates, self._compute_sublayer(gates, workers)):
+                self._commit(i, out, start)
             return
         pool = _worker_pool(workers)
         start = time.time()
@@ -197,6 +206,52 @@ class AdaptiveMPS(MatrixProductStatePyTorch):
             for t in out[0:2]:
                 t.record_stream(ready)
             self._commit(i, out, start)
+
+    def _compute_sublayer(self, gates, workers: int = 1):
+        """Results of the truncated-SVD methods for gates on disjoint bonds (no
+        state change).  Theta and the randomized factorizations run per bond,
+        concurrently on `workers` host threads with their own HIP streams; the
+        exact Gram factorizations left over are batched on the current stream."""
+        from . import fastsvd
+
+        def prep(i, U4):
+            A, B = self.tensors[i], self.tensors[i + 1]
+            cl, cm, cr = A.shape[0], A.shape[2], B.shape[2]
+            local = self.dtype_policy.default
+            r = fastsvd.prepare_update(self, i, A.to(dtype=local), B.to(dtype=local), _gate4(U4),
+                                       self.chi_max_per_bond[i],
+                                       self._budget_ok_factory(cl, cm, cr, local), local)
+            return [r, cm]
+
+        if workers > 1 and len(gates) > 1:
+            pool = _worker_pool(workers)
+            ready = torch.cuda.current_stream(self.device)
+            ready_ev = ready.record_event()
+
+            def job(i, U4):
+                st = _thread_stream(self.device)
+                with torch.cuda.stream(st):
+                    st.wait_event(ready_ev)
+                    out = prep(i, U4)
+                    done = st.record_event()
+                return out, done
+
+            outs = []
+            for f in [pool.submit(job, i, U) for i, U in gates]:
+                out, done = f.result()
+                ready.wait_event(done)
+                r = out[0]
+                held = ([r.X] + ([r.feats] if r.feats is not None else [])
+                        if isinstance(r, fastsvd.GramJob) else list(r[0:2]))
+                for t in held:          # produced on a worker stream, consumed on this one
+                    t.record_stream(ready)
+                outs.append(out)
+        else:
+            outs = [prep(i, U) for i, U in gates]
+        jobs = [j for j, (r, _) in enumerate(outs) if isinstance(r, fastsvd.GramJob)]
+        for j, res in zip(jobs, fastsvd.finish_gram(self, [outs[j][0] for j in jobs])):
+            outs[j][0] = res
+        return [tuple(r) + (cm,) for r, cm in outs]
 
     def _compute_update(self, i: int, U4):
         """Theta, factorization, rank rule and rebuild for bond i; no state change."""

@@ -2,8 +2,18 @@
 
 Orders up to 64 run in the library's batched FP64 Jacobi kernel
 (``aq_heev_small``: one work-group, matrix in LDS, no launch-latency chain);
-larger orders use rocSOLVER's divide-and-conquer through ``torch.linalg.eigh``.
+larger orders use rocSOLVER's divide-and-conquer, either through
+``torch.linalg.eigh`` or, for a batch of cores (all Gram matrices of a
+brickwork sub-layer), through the library's strided-batched
+``aq_heevd_batched``, which pays rocSOLVER's kernel chain once per batch.
 Both return eigenvalues in descending order with matching eigenvector columns.
+
+rocSOLVER can report non-convergence on strongly clustered spectra (many
+eigenvalues at zero, as in the Gram matrix of a rank-deficient theta).  As
+the reference's SVD cascade retries with a 1e-12 jitter of the input
+(linalg_robust.py:31-39), the eigensolver retries once on G + E, E a random
+Hermitian matrix of norm ~1e-13 ||G||_max, which separates the cluster and
+moves every eigenvalue by at most ||E||_2.
 """
 from __future__ import annotations
 
@@ -12,10 +22,36 @@ import torch
 JACOBI_MAX = 64
 
 
+def heevd_batched(G: torch.Tensor):
+    """Eigenpairs of Hermitian complex128 matrices G [batch, n, n] (n > 64) in
+    one rocSOLVER strided-batched call: (eigenvalues desc [batch, n],
+    eigenvectors as columns [batch, n, n]).  Raises if any matrix did not
+    converge; G is not modified."""
+    from . import kernels as K
+    from ._ffi import check, lib
+    A = G.to(torch.complex128).contiguous().clone()
+    bsz, n, _ = A.shape
+    W = torch.empty((bsz, n), dtype=torch.float64, device=A.device)
+    E = torch.empty((bsz, n), dtype=torch.float64, device=A.device)
+    info = torch.zeros(bsz, dtype=torch.int32, device=A.device)
+    check(lib().aq_heevd_batched(n, K._ptr(A), n * n, K._ptr(W), n, K._ptr(E), K._ptr(info),
+                                 bsz, K._stream(A)), "aq_heevd_batched")
+    if int(info.abs().max()) != 0:
+        raise RuntimeError(f"aq_heevd_batched: not converged (info {info.tolist()})")
+    # row j of A holds conj(v_j): V = A^H, columns ascending -> flip to descending
+    return W.flip(-1), A.transpose(-2, -1).conj().flip(-1)
+
+
 def eigh_desc(G: torch.Tensor):
     n = G.shape[-1]
     if n <= JACOBI_MAX:
         from .rank_mlp import heev_small
         return heev_small(G)
-    lam, V = torch.linalg.eigh(G)
+    try:
+        lam, V = torch.linalg.eigh(G)
+    except RuntimeError:
+        scale = 1e-13 * float(torch.diagonal(G, dim1=-2, dim2=-1).real.abs().max())
+        E = torch.randn(G.shape, dtype=G.dtype, device=G.device)
+        E = (E + E.transpose(-2, -1).conj()) * (0.5 * scale / n ** 0.5)
+        lam, V = torch.linalg.eigh(G + E)
     return lam.flip(-1), V.flip(-1)

@@ -55,6 +55,9 @@ class FactorStats:
             self.rejects[reason] = self.rejects.get(reason, 0) + 1
 
 
+RSVD_ATTEMPTS = 2      # a rejected sketch is retried once at twice the width
+RETRY_REASONS = ("tail", "uncertified", "rank_at_width", "orth")
+
 _DEFAULT = {}
 _INIT_LOCK = __import__("threading").Lock()
 
@@ -77,22 +80,25 @@ def default_predictor(device):
     return _DEFAULT[key]
 
 
-def _gram_factors(X: torch.Tensor):
-    """Exact leading factors of X via the FP64 Gram matrix of the smaller side.
-    Returns (U [m, r] c128, S [r] f64 descending, Vh [r, n] c128, X64)."""
+def _gram_matrix(X: torch.Tensor):
+    """FP64 Gram matrix of the smaller side of X: (G, left, X64) with left
+    True for G = X X^H [m, m], False for G = X^H X [n, n]."""
     m, n = X.shape
     X64 = X.to(torch.complex128)
     if m <= n:
         with RL.timed("gram_fp64_cgemm", 8.0 * m * m * n, "fp64_mfma"):
-            G = K.cgemm(X64, X64.T, conj_b=True)         # X X^H  [m, m]
-        lam, U = eigh_desc(G)
-        S = torch.sqrt(torch.clamp(lam, min=0.0))
-        return U, S, None, X64
+            return K.cgemm(X64, X64.T, conj_b=True), True, X64
     with RL.timed("gram_fp64_cgemm", 8.0 * n * n * m, "fp64_mfma"):
-        G = K.cgemm(X64.T, X64, conj_a=True)             # X^H X  [n, n]
-    lam, Vr = eigh_desc(G)
+        return K.cgemm(X64.T, X64, conj_a=True), False, X64
+
+
+def _gram_factors(X: torch.Tensor):
+    """Exact leading factors of X via the FP64 Gram matrix of the smaller side.
+    Returns (U [m, r] c128, S [r] f64 descending, Vh [r, n] c128, X64)."""
+    G, left, X64 = _gram_matrix(X)
+    lam, V = eigh_desc(G)
     S = torch.sqrt(torch.clamp(lam, min=0.0))
-    return None, S, Vr, X64
+    return (V, S, None, X64) if left else (None, S, V, X64)
 
 
 def _finish_from_gram(mps, X, U, S, Vr, X64, k_dtype, cap, budget_ok, cl, cr, eps):
@@ -124,6 +130,22 @@ def _finish_from_gram(mps, X, U, S, Vr, X64, k_dtype, cap, budget_ok, cl, cr, ep
 def two_site_update(mps, i: int, A: torch.Tensor, B: torch.Tensor, gate: np.ndarray, cap: int,
                     budget_ok: Callable[[int], bool], local: torch.dtype):
     """Returns (A_new, B_new, k, eps_local, entropy, condS, driver, dtype)."""
+    r = prepare_update(mps, i, A, B, gate, cap, budget_ok, local)
+    return r if not isinstance(r, GramJob) else finish_gram(mps, [r])[0]
+
+
+class GramJob:
+    """An update waiting for the exact Gram factorization."""
+
+    def __init__(self, X, cap, budget_ok, local, cl, cr, feats):
+        self.X, self.cap, self.budget_ok, self.local = X, cap, budget_ok, local
+        self.cl, self.cr, self.feats = cl, cr, feats
+
+
+def prepare_update(mps, i: int, A: torch.Tensor, B: torch.Tensor, gate: np.ndarray, cap: int,
+                   budget_ok: Callable[[int], bool], local: torch.dtype):
+    """Theta and, for the randomized methods, the certified randomized SVD.
+    Returns the update's result tuple, or a GramJob for ``finish_gram``."""
     method = mps.svd_method
     cl, cr = A.shape[0], B.shape[2]
     st = _factor_stats(mps)
@@ -145,24 +167,79 @@ def two_site_update(mps, i: int, A: torch.Tensor, B: torch.Tensor, gate: np.ndar
             pred = mps.rank_predictor = default_predictor(X.device)
     if pred is not None and (method == "learned" or getattr(mps, "harvest", None) is not None):
         feats = pred.features(mps, i, X, gate, cap)
-    out = None
     if method in ("rsvd", "learned"):
         from . import rsvd
         if method == "learned" and pred is not None and pred.trained:
             k_guess = pred.predict_width(feats, cap, min(X.shape))
         else:
             k_guess = min(cap, 2 * cm)          # operator-Schmidt growth bound, no model
-        out = rsvd.try_rsvd(mps, i, X, cap, budget_ok, local, cl, cr, st, k_guess)
-        if out is None:
-            st.bump("rsvd_fallback")
+        out = None
+        for attempt in range(RSVD_ATTEMPTS):
+            try:
+                out, why = rsvd.try_rsvd(mps, i, X, cap, budget_ok, local, cl, cr, st, k_guess)
+            except RuntimeError:        # core eigensolver did not converge
+                st.reject("eigensolver")
+                break
+            if out is not None or why not in RETRY_REASONS:
+                break
+            k_guess *= 2                # a wider sketch, fresh random columns
+        if out is not None:
+            _harvest(mps, feats, out[2])
+            return out
+        st.bump("rsvd_fallback")
     elif method != "gram":
         raise ValueError(f"unknown svd_method {method!r}")
-    if out is None:
-        U, S, Vr, X64 = _gram_factors(X)
-        A_new, B_new, k, e, s, c, dt = _finish_from_gram(mps, X, U, S, Vr, X64, local, cap,
-                                                        budget_ok, cl, cr, mps.eps_bond)
-        st.bump("gram")
-        out = (A_new, B_new, k, e, s, c, "gram_fp64", dt)
+    return GramJob(X, cap, budget_ok, local, cl, cr, feats)
+
+
+def _harvest(mps, feats, k):
     if feats is not None and getattr(mps, "harvest", None) is not None:
-        mps.harvest.append((feats.cpu().numpy(), out[2]))
-    return out
+        mps.harvest.append((feats.cpu().numpy(), k))
+
+
+def finish_gram(mps, jobs):
+    """Exact Gram factorization of several updates.  Gram matrices of equal
+    order are factored together: orders <= 64 in the batched Jacobi kernel,
+    larger ones in one strided-batched rocSOLVER call per order."""
+    from .eig import JACOBI_MAX, heevd_batched
+    st = _factor_stats(mps)
+    grams = [_gram_matrix(j.X) for j in jobs]
+    eig = [None] * len(jobs)
+    orders = {}
+    for idx, (G, _, _) in enumerate(grams):
+        orders.setdefault(G.shape[0], []).append(idx)
+    for n, idxs in orders.items():
+        try:
+            if n <= JACOBI_MAX or len(idxs) == 1:
+                for idx in idxs:
+                    eig[idx] = eigh_desc(grams[idx][0])
+            else:
+                lam, V = heevd_batched(torch.stack([grams[idx][0] for idx in idxs]))
+                for b, idx in enumerate(idxs):
+                    eig[idx] = (lam[b], V[b])
+        except RuntimeError:
+            pass                        # per-matrix retry below
+    outs = []
+    for idx, j in enumerate(jobs):
+        G, left, X64 = grams[idx]
+        if eig[idx] is None:
+            try:
+                eig[idx] = eigh_desc(G)
+            except RuntimeError:
+                # the FP64 eigensolver failed twice: the plain method's SVD cascade
+                U, S, Vh, driver = robust_svd(j.X)
+                k, e, s, c = choose_rank_from_sigma(S, mps.eps_bond, j.cap, j.budget_ok)
+                A_new, B_new = K.rebuild(U, S, Vh, k, j.cl, j.cr)
+                st.bump("plain")
+                outs.append((A_new, B_new, k, e, s, c, driver, j.local))
+                _harvest(mps, j.feats, k)
+                continue
+        lam, V = eig[idx]
+        S = torch.sqrt(torch.clamp(lam, min=0.0))
+        U, Vr = (V, None) if left else (None, V)
+        A_new, B_new, k, e, s, c, dt = _finish_from_gram(mps, j.X, U, S, Vr, X64, j.local, j.cap,
+                                                        j.budget_ok, j.cl, j.cr, mps.eps_bond)
+        st.bump("gram")
+        outs.append((A_new, B_new, k, e, s, c, "gram_fp64", dt))
+        _harvest(mps, j.feats, k)
+    return outs

@@ -1,6 +1,6 @@
 """Randomized truncated SVD of the theta matrix, certified in FP64.
 
-For X [m, n] (complex64) and a width l = k_guess + p, p = max(16, k_guess/2):
+For X [m, n] (complex64) and a width l = k_guess + p, p = max(16, k_guess):
   Y = X Omega                      range finder (MFMA GEMM, BF16x3 optional)
   q x { Z = X^H Q ; Y = X Z' }     optional power steps (q = POWER_STEPS),
                                    re-orthonormalised
@@ -15,15 +15,10 @@ sum_{i<=j} sigma_i^2 <= E_j + tail with E_j = sum_{i<=j} lam_i, while
 (i)  tail <= FID_TOL * ||X||_F^2: the kept weight of the rank-k result is then
      within FID_TOL of the exact truncation's (sum_{i<k} (sigma_i^2 - lam_i)
      <= tail), i.e. the truncation fidelity agrees to 1e-7 per update; and
-(ii) the rank is certified (``certify_rank``): every spectrum consistent with
-     the bounds crosses the rule's threshold (1 - eps^2) ||X||^2, widened by
-     one float32 ulp of ||X||^2 (the resolution of the rule's own float32
-     arithmetic), within MAX_SPREAD consecutive ranks, and the rank the rule
-     returns on the captured spectrum is one of them.  Where the exact partial
-     sums sit farther than that resolution from the threshold this is the
-     exact rule's rank; inside it the reference's float32 rule itself cannot
-     separate the candidates, whose weights all lie within the band (so the
-     kept weight, and the fidelity, differ by at most 2 ulp + tail).
+(ii) the rank is certified (``certify_rank``): the reference's float32 rule,
+     evaluated in its own arithmetic, returns the same rank for every
+     spectrum consistent with the bounds, and the rank kernel returned it on
+     the captured spectrum.
 Otherwise the caller falls back to the exact Gram factorization.
 """
 from __future__ import annotations
@@ -39,43 +34,93 @@ from .eig import eigh_desc
 from .truncation import choose_rank_from_sigma
 
 FID_TOL = 1e-7         # admissible uncaptured weight, of ||X||_F^2
-POWER_STEPS = 0        # power iterations; oversampling k/2 captures decaying spectra
+POWER_STEPS = 0        # power iterations; 100% oversampling captures decaying spectra
 WIDE_FRACTION = 0.5    # widths beyond this share of min(m, n) go to the Gram path
-RULE_BAND = 2.0 ** -23 # float32 resolution of the rank rule's partial sums, of ||X||_F^2
-MAX_SPREAD = 2         # certified ranks may differ by this much (all within RULE_BAND)
 
 
-def certify_rank(lam: np.ndarray, tail: float, total: float, eps: float, cap: int):
-    """Ranks the rule (truncation.py:52-63) can return for any spectrum with
-    sigma_i^2 >= lam_i and partial sums <= E_j + tail, total fixed.
+def rule_rank_f32(S: np.ndarray, eps: float) -> int:
+    """k_tol of the reference's rule (truncation.py:52-61) in its float32
+    arithmetic: float32 squares, cumulative sums accumulated in double and
+    rounded to float32, threshold fl32(fl32(1 - eps^2) * total), left search."""
+    S = np.asarray(S, np.float32)
+    E = np.cumsum((S * S).astype(np.float64)).astype(np.float32)
+    thresh = np.float32(np.float32(1.0 - eps * eps) * E[-1])
+    return min(int(np.searchsorted(E, thresh, side="left")) + 1, len(S))
 
-    Returns (k_lo, k_hi) with k_hi - k_lo <= MAX_SPREAD (after the cap), or
-    None when the crossing is not pinned that closely within the captured
-    range.  FP64 throughout; lam descending."""
-    E = np.cumsum(np.asarray(lam, np.float64))
-    thresh = (1.0 - eps * eps) * total
-    band = RULE_BAND * total
-    early = np.nonzero(E + tail >= thresh - band)[0]     # largest admissible sums
-    k_early = int(early[0]) + 1 if early.size else len(E) + 1
-    if k_early > cap:
-        return (cap, cap)            # no admissible spectrum reaches the threshold by cap
-    late = np.nonzero(E >= thresh + band)[0]             # smallest admissible sums
+
+def _rule_weights(x: np.ndarray) -> np.ndarray:
+    """Per-value weights as the rule forms them: fl32(fl32(sqrt(x))^2)."""
+    s = np.sqrt(np.maximum(x, 0.0)).astype(np.float32)
+    return (s * s).astype(np.float64)
+
+
+def _threshold(total_double: float, eps: float) -> np.float32:
+    return np.float32(np.float32(1.0 - eps * eps) * np.float32(total_double))
+
+
+def certify_rank(lam: np.ndarray, tail: float, eps: float, cap: int, total: float = None):
+    """Rank the float32 rule returns for every spectrum consistent with the
+    captured bounds, or None if it is not pinned.
+
+    The true squared singular values satisfy sigma_i^2 >= lam_i and
+    sum_i (sigma_i^2 - lam_i) + sum_{i >= l} sigma_i^2 = tail (up to the FP64
+    error of lam, added to both sides).  In the rule's arithmetic a value whose
+    interval [lam_i - d, lam_i + tail + d] rounds to one float32 weight has
+    that weight exactly; any other value's weight exceeds the lower one by at
+    most its share of the tail plus three float32 ulps.  This bounds every
+    float32 partial sum; the float32 total is the exact weight of the stable
+    values plus the remaining exact weight (||X||^2 minus theirs, known to
+    the tail) within two float32 roundings, which bounds the threshold; the rank
+    is certified when the earliest and the latest admissible crossings agree
+    after the cap.  The rule is monotone in the partial sums and in the
+    threshold, so every admissible spectrum then yields that rank."""
+    lam = np.asarray(lam, np.float64)
+    if total is None:
+        total = float(lam.sum() + tail)
+    d = 1e-13 * total                                  # FP64 error of lam and of sums
+    tail_e = tail + d
+    w_lo = _rule_weights(lam - d)
+    w_hi = _rule_weights(lam + tail_e)
+    unstable = w_hi != w_lo
+    slack = np.where(unstable, 3.0 * np.spacing(w_hi.astype(np.float32)).astype(np.float64), 0.0)
+    L = np.cumsum(w_lo)
+    U = L + tail_e + np.cumsum(slack)
+    # float32 total: stable values contribute their weight exactly; the rest of
+    # the exact total (unstable values and everything beyond the captured
+    # width) contributes its weight within two float32 roundings
+    stable = ~unstable
+    base = float(w_lo[stable].sum())
+    rest = total - float(lam[stable].sum())
+    rest_lo = max(rest - (tail_e if stable.any() else 0.0), 0.0)
+    rest_hi = rest + d * len(lam)
+    thr_lo = _threshold(base + rest_lo * (1.0 - 2.0 ** -22) - d, eps)
+    thr_hi = _threshold(base + rest_hi * (1.0 + 2.0 ** -22) + d, eps)
+    early = np.nonzero(U.astype(np.float32) >= thr_lo)[0]
+    k_min = int(early[0]) + 1 if early.size else len(lam) + 1
+    if k_min > cap:
+        return cap                  # no admissible spectrum reaches the threshold by cap
+    late = np.nonzero(L.astype(np.float32) >= thr_hi)[0]
     if not late.size:
         return None
-    k_late = int(late[0]) + 1
-    lo, hi = min(k_early, cap), min(k_late, cap)
-    return (lo, hi) if hi - lo <= MAX_SPREAD else None
+    k_min, k_max = min(k_min, cap), min(int(late[0]) + 1, cap)
+    return k_min if k_min == k_max else None
 
 
 def orth(Y: torch.Tensor, infos: list) -> torch.Tensor:
-    """Orthonormal basis of the columns of Y (complex128) by shifted Cholesky-QR
-    x2, without a host round trip: the factorizations' info codes are appended
-    to `infos` and checked once by the caller."""
-    for it in range(2):
+    """Orthonormal basis of the columns of Y [m, l] (complex128) by shifted
+    Cholesky-QR followed by Cholesky-QR2 (shiftedCholeskyQR3: Fukaya,
+    Kannan, Nakatsukasa, Yamamoto, Yanagisawa, SIAM J. Sci. Comput. 42 (2020)),
+    with their shift 11 (m l + l (l + 1)) u ||Y||^2 (||Y||_F^2 as the bound).
+    The first pass keeps the span of Y and makes the next two well posed.  No
+    host round trip: the factorizations' info codes are appended to `infos`
+    and checked once by the caller."""
+    m, l = Y.shape
+    u = 2.0 ** -53
+    for it in range(3):
         W = K.cgemm(Y.T, Y, conj_a=True)
         if it == 0:
-            shift = 1e-13 * torch.diagonal(W).real.sum()
-            W = W + shift * torch.eye(W.shape[0], dtype=W.dtype, device=W.device)
+            shift = 11.0 * (m * l + l * (l + 1)) * u * torch.diagonal(W).real.sum()
+            W = W + shift * torch.eye(l, dtype=W.dtype, device=W.device)
         L, info = torch.linalg.cholesky_ex(W)
         infos.append(info)
         Y = torch.linalg.solve_triangular(L.conj().T, Y, upper=True, left=False)
@@ -102,17 +147,18 @@ def width_for(k_guess: int, m: int, n: int, cap: int):
     """Range-finder width for a rank guess: k_guess plus oversampling."""
     r = min(m, n)
     k_guess = max(1, min(k_guess, cap, r))
-    p = max(16, k_guess // 2)
+    p = max(16, k_guess)
     return k_guess, min(r, k_guess + p)
 
 
 def try_rsvd(mps, i, X, cap, budget_ok, local, cl, cr, st, k_guess):
+    """One certified randomized factorization at width k_guess + p.
+    Returns (result tuple, None) or (None, reason for the fallback)."""
     m, n = X.shape
     k_guess, l = width_for(k_guess, m, n, cap)
-    st.predicted.append((k_guess, None))
     if l >= WIDE_FRACTION * min(m, n):
         st.reject("wide")
-        return None
+        return None, "wide"
     prec = mps.theta_precision
     gen = _generator(mps, X.device)
     Om = torch.randn(n, l, dtype=torch.complex64, device=X.device, generator=gen)
@@ -138,7 +184,7 @@ def try_rsvd(mps, i, X, cap, budget_ok, local, cl, cr, st, k_guess):
     lam_h, total, bad = host[:-2], float(host[-2]), host[-1] != 0
     if bad or not np.isfinite(host).all():
         st.reject("orth")
-        return None
+        return None, "orth"
     tail = max(total - float(lam_h.sum()), 0.0)
     S = torch.sqrt(lam)
     S_aug = torch.cat([S, torch.tensor([math.sqrt(tail)], dtype=S.dtype, device=S.device)])
@@ -147,23 +193,24 @@ def try_rsvd(mps, i, X, cap, budget_ok, local, cl, cr, st, k_guess):
     eps = mps.eps_bond
     S_rule = S_aug.to(torch.float32) if local == torch.complex64 else S_aug
     k, eps_loc, entropy, condS = choose_rank_from_sigma(S_rule, eps, cap, budget_ok)
-    st.predicted[-1] = (k_guess, k)
-    st.tails.append(tail / total if total > 0 else 0.0)
+    with st.lock:
+        st.predicted.append((k_guess, k))
+        st.tails.append(tail / total if total > 0 else 0.0)
     if k >= l:
         st.reject("rank_at_width")
-        return None
+        return None, "rank_at_width"
     if tail > FID_TOL * total:
         st.reject("tail")
-        return None
-    rng = certify_rank(lam_h, tail, total, eps, cap)
-    # a rank below the certified pair comes from the memory budget, which
+        return None, "tail"
+    k_cert = certify_rank(lam_h, tail, eps, cap, total)
+    # a rank below the certified one comes from the memory budget, which
     # depends on k only and binds identically on the exact spectrum
-    if rng is None or k > rng[1] or (k < rng[0] and mps.budget_global_mb is None):
+    if k_cert is None or k > k_cert or (k < k_cert and mps.budget_global_mb is None):
         st.reject("uncertified")
-        return None
+        return None, "uncertified"
     if math.isfinite(condS) and condS > mps.dtype_policy.promote_if_cond_gt:
         st.reject("condition")
-        return None  # promotion semantics: let the exact path decide in FP64
+        return None, "condition"  # promotion semantics: let the exact path decide in FP64
     Sk = S[:k]
     inv = torch.where(Sk > 0, 1.0 / Sk, torch.zeros_like(Sk))
     Uk = K.cgemm(Q, UB[:, :k].contiguous())                       # [m, k]
@@ -171,4 +218,4 @@ def try_rsvd(mps, i, X, cap, budget_ok, local, cl, cr, st, k_guess):
     A_new, B_new = K.rebuild(Uk.to(local), Sk.to(torch.float32), Vh.to(local).contiguous(),
                              k, cl, cr)
     st.bump("rsvd")
-    return A_new, B_new, k, eps_loc, entropy, condS, "rsvd_fp64_core", local
+    return (A_new, B_new, k, eps_loc, entropy, condS, "rsvd_fp64_core", local), None

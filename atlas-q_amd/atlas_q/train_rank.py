@@ -10,7 +10,10 @@ rule kept.  Circuit families: Haar brickwork, weakly entangling brickwork
 (exp(-i a H), H Hermitian Gaussian, a in [0.03, 0.6]) and the RY/RZ + CZ
 hardware-efficient ansatz, over chain lengths 8-40, caps 8-256 and cutoffs
 1e-2 .. 1e-8; and steady-state slices at bond 32-256 with flat or decaying
-Schmidt profiles (``atlas_q.workloads``) under Haar or weak gates.  Split by circuit (90/10), as the reference's dataset builder
+Schmidt profiles (``atlas_q.workloads``) at bond 32-1024 under Haar or weak
+gates (bond >= 512 harvested with the exact Gram method, whose ranks are the
+plain method's up to float32 rounding).  Split by circuit (90/10), as the
+reference's dataset builder
 splits by file (scripts/ai_tools/build_ft_dataset.py:103-108).
 """
 from __future__ import annotations
@@ -42,14 +45,16 @@ def slice_job(rng):
     """Steady-state slices (bonds chi at both ends) with flat or decaying
     Schmidt profiles, Haar or weakly entangling gates."""
     from .workloads import gate_schedule, slice_tensors
-    chi = int(2 ** rng.integers(5, 9))
+    chi = int(2 ** rng.integers(5, 11))
     kind = "flat" if rng.random() < 0.3 else "decay"
     tau = float(rng.uniform(3.0, 40.0))
     angle = None if rng.random() < 0.3 else float(rng.uniform(0.02, 0.6))
     eps = float(10.0 ** -rng.integers(2, 9))
-    sites = int(rng.integers(4, 7))
+    big = chi >= 512
+    sites = 4 if big else int(rng.integers(4, 7))
     ts = slice_tensors(sites, chi, kind, seed=int(rng.integers(1 << 30)), tau=tau)
-    gates = gate_schedule(sites, int(rng.integers(4, 10)), int(rng.integers(1 << 30)), angle)
+    gates = gate_schedule(sites, int(rng.integers(3, 7) if big else rng.integers(4, 10)),
+                          int(rng.integers(1 << 30)), angle)
     return chi, eps, ts, gates
 
 
@@ -73,12 +78,15 @@ def harvest(pairs: int, seed: int = 0, max_seconds: float = 600.0, device="cuda"
     pred = RankPredictor(device)
     F, K, C = [], [], []
     cid = 0
-    t0 = time.time()
+    t0 = last = time.time()
     while len(K) < pairs and time.time() - t0 < max_seconds:
         if cid % 4 == 3:
             cap, eps, ts, gates = slice_job(rng)
+            # bond >= 512: the exact FP64 Gram factorization stands in for the
+            # float32 SVD (same rule, same ranks up to float32 rounding, 10x faster)
             m = AdaptiveMPS(len(ts), bond_dim=1, eps_bond=eps, chi_max_per_bond=cap,
-                            device=device, svd_method="plain", rank_predictor=pred)
+                            device=device, svd_method="gram" if cap >= 512 else "plain",
+                            rank_predictor=pred)
             m.tensors = ts
             m.bond_dims = [cap] * (len(ts) - 1)
             ops = [("2q", b, U) for b, U in gates]
@@ -98,6 +106,9 @@ def harvest(pairs: int, seed: int = 0, max_seconds: float = 600.0, device="cuda"
             K.append(k)
             C.append(cid)
         cid += 1
+        if time.time() - last > 30.0:
+            last = time.time()
+            print(f"  harvest: {len(K)} pairs, {cid} circuits, {last - t0:.0f}s", flush=True)
     return np.array(F, np.float32), np.array(K, np.int64), np.array(C, np.int64)
 
 

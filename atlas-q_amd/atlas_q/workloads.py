@@ -7,8 +7,9 @@ Site tensors are Lambda_{q-1} G_q with G_q a random right-isometry
 
 * "flat":  weights of a saturated random circuit (Marchenko-Pastur-like,
            sigma_i ~ sqrt(1 - i/chi)), where the rank rule keeps the cap;
-* "decay": sigma_i = exp(-i / tau), a low-entanglement state where the kept
-           rank is far below chi (the regime the learned width targets).
+* "decay": sigma_i = exp(-i / tau) + 1e-6 exp(-i / chi), a low-entanglement
+           state where the kept rank is far below chi (the regime the learned
+           width targets).
 """
 from __future__ import annotations
 
@@ -21,7 +22,10 @@ def schmidt_profile(chi: int, kind: str = "flat", tau: float = 24.0) -> np.ndarr
     if kind == "flat":
         s = np.sqrt(np.clip(1.0 - i / chi, 1e-3, None))
     else:
-        s = np.exp(-i / tau)
+        # exponential decay onto a floor at 1e-6 sigma_0 (distinct values): a
+        # truncated float32 state holds no Schmidt weight below ~1e-7 of the
+        # largest, and an unfloored profile reaches float32 subnormals
+        s = np.exp(-i / tau) + 1e-6 * np.exp(-i / chi)
     return s / np.linalg.norm(s)
 
 

@@ -59,6 +59,8 @@ def parse():
     ap.add_argument("--cpu-updates", type=int, default=2)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-low", action="store_true",
+                    help="skip the low-entanglement (learned-width) measurement")
     ap.add_argument("--seed", type=int, default=2025)
     return ap.parse_args()
 
@@ -126,12 +128,13 @@ def make_mps(torch, tensors, chi, method, prec):
 
 
 def fidelity_vs_fp64(torch, A, B, gate, chi, methods):
-    """Truncation fidelity of each method's output against the FP64 truncation."""
+    """Truncation fidelity of each method's output against the FP64 truncation
+    of the same theta at the smallest kept rank among the methods."""
     from atlas_q import kernels as K
     A64, B64 = A.to(torch.complex128), B.to(torch.complex128)
     X = K.theta(A64, B64, gate)
     U, S, Vh = torch.linalg.svd(X, full_matrices=False)
-    k = min(chi, S.numel())
+    k = min(chi, S.numel(), min(An.shape[2] for An, _ in methods.values()))
     P64 = (U[:, :k] * S[:k]) @ Vh[:k]
     out = {}
     for name, (An, Bn) in methods.items():
@@ -142,6 +145,47 @@ def fidelity_vs_fp64(torch, A, B, gate, chi, methods):
         kept = float((P.norm() ** 2 / X.norm() ** 2).item())
         out[name] = {"F": F, "kept_weight": kept}
     out["fp64_kept_weight"] = float((S[:k] ** 2).sum() / (S ** 2).sum())
+    return out
+
+
+def low_entanglement(torch, a, dev):
+    """The learned width where it applies: the first brickwork layer of weak
+    gates (exp(-0.1 i H)) on a fresh slice at bond CHI whose Schmidt weights
+    decay (tau = 10), cutoff 1e-8: kept ranks ~100 of 1024.  All three
+    methods on the same slice and gates, sub-layer concurrency as above."""
+    from atlas_q.adaptive_mps import _gate4
+    from atlas_q.workloads import gate_schedule as weak_schedule
+    from atlas_q.workloads import slice_tensors
+    chi, sites = a.chi, a.sites
+    ts = slice_tensors(sites, chi, "decay", seed=a.seed + 7, tau=10.0, device=dev)
+    gates = weak_schedule(sites, (sites - 1 + 1) // 2, seed=a.seed + 8, angle=0.1)
+    out = {"workload": f"first layer of weak gates exp(-0.1iH) on a {sites}-site slice at bond "
+                       f"{chi}, Schmidt weights exp(-2i/10), cutoff 1e-8, complex64",
+           "updates": len(gates)}
+    ranks = {}
+    for name, meth, prec in (("learned", a.method, a.theta_precision), ("gram", "gram", a.theta_precision),
+                             ("plain", "plain", "native")):
+        run_gates(torch, make_mps(torch, ts, chi, meth, prec), gates, a.workers)   # warm-up
+        m = make_mps(torch, ts, chi, meth, prec)
+        t = run_gates(torch, m, gates, a.workers)
+        out[f"{name}_updates_per_s"] = len(gates) / t
+        ranks[name] = m.statistics.logs["k_star"]
+        if hasattr(m, "factor_stats"):
+            out[f"{name}_factor_counts"] = dict(m.factor_stats.counts)
+            if m.factor_stats.rejects:
+                out[f"{name}_rejects"] = dict(m.factor_stats.rejects)
+    out["kept_ranks"] = ranks
+    out["ranks_equal_to_plain"] = {k: v == ranks["plain"] for k, v in ranks.items()}
+    out["speedup_learned_vs_plain"] = out["learned_updates_per_s"] / out["plain_updates_per_s"]
+    b, U = gates[0]
+    outs = {}
+    for name, meth, prec in (("fast", a.method, a.theta_precision), ("plain", "plain", "native")):
+        mm = make_mps(torch, ts, chi, meth, prec)
+        mm.apply_two_site_gate(b, U)
+        outs[name] = (mm.tensors[b], mm.tensors[b + 1])
+    f = fidelity_vs_fp64(torch, ts[b], ts[b + 1], _gate4(U), chi, outs)
+    out["fidelity"] = f
+    out["abs_dF_learned_vs_plain"] = abs(f["fast"]["F"] - f["plain"]["F"])
     return out
 
 
@@ -208,6 +252,8 @@ def main():
             fid.append(fidelity_vs_fp64(torch, state[b], state[b + 1], g, chi, outs))
     dF = max((abs(f["fast"]["F"] - f["plain"]["F"]) for f in fid), default=None)
 
+    low = None if a.no_low else low_entanglement(torch, a, dev)
+
     # ---------------------------------------------------------------- CPU baseline
     cpu = None
     if not a.no_cpu and a.cpu_updates > 0:
@@ -246,7 +292,9 @@ def main():
                        "speedup": (fast_seq / plain_seq) if fast_seq and plain_seq else None},
         "kept_ranks": sorted(set(ks)),
         "fidelity": {"max_abs_dF_fast_vs_plain": dF, "samples": fid},
-        "factor_counts": getattr(fast, "factor_stats", None).counts if hasattr(fast, "factor_stats") else None,
+        "factor_counts": fast.factor_stats.counts if hasattr(fast, "factor_stats") else None,
+        "factor_rejects": fast.factor_stats.rejects if hasattr(fast, "factor_stats") else None,
+        "low_entanglement": low,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -139,6 +139,17 @@ int aq_heev_small(int64_t n, const void* A, int64_t lda, int64_t strideA, double
                   int64_t ldv, int64_t strideV, int64_t batch, int max_sweeps, double tol,
                   int* sweeps, hipStream_t stream);
 
+/* Batched FP64 Hermitian eigen-decomposition of the truncated SVD's Gram
+ * cores (no reference counterpart: the reference factors theta with a thin
+ * SVD, adaptive_mps.py:249 via linalg_robust.py:17-52; the Gram method needs
+ * the eigenpairs of X X^H).  rocSOLVER zheevd, strided-batched.  A: complex128
+ * row-major n x n Hermitian matrices at A + b*strideA (elements), overwritten:
+ * row j of matrix b becomes the conjugate of the j-th eigenvector.  W: float64
+ * eigenvalues, ascending, at W + b*strideW.  E: float64 workspace [batch*n].
+ * info: int32 [batch] on the device, 0 on success (> 0: not converged). */
+int aq_heevd_batched(int64_t n, void* A, int64_t strideA, double* W, int64_t strideW, double* E,
+                     int32_t* info, int64_t batch, hipStream_t stream);
+
 /* Rank-predictor MLP 32 -> 128 -> 128 -> 1 with GELU and BF16 weights (no
  * reference counterpart for pre-SVD prediction; the reference's predictors,
  * src/atlas_q/tools_qih/ai_rank_predictor.py:14-89 and

@@ -36,6 +36,22 @@ def test_heev_small_matches_lapack(n):
         assert np.allclose(np.conj(V[b].T) @ V[b], np.eye(n), atol=1e-12)
 
 
+@pytest.mark.parametrize("n,batch", [(65, 3), (130, 2), (512, 4)])
+def test_heevd_batched_matches_lapack(n, batch):
+    from atlas_q.eig import heevd_batched
+    rng = np.random.default_rng(n)
+    Z = rng.standard_normal((batch, n, n)) + 1j * rng.standard_normal((batch, n, n))
+    G = Z @ np.conj(np.transpose(Z, (0, 2, 1)))
+    lam, V = heevd_batched(torch.from_numpy(G).to(DEV))
+    lam, V = lam.cpu().numpy(), V.cpu().numpy()
+    ref = np.linalg.eigvalsh(G)[:, ::-1]
+    scale = np.abs(ref).max()
+    assert np.abs(lam - ref).max() < 1e-11 * scale
+    for b in range(batch):
+        assert np.abs(G[b] @ V[b] - V[b] * lam[b]).max() < 1e-10 * scale
+        assert np.abs(np.conj(V[b].T) @ V[b] - np.eye(n)).max() < 1e-10
+
+
 def _run(method, n=12, depth=10, chi=32, eps=1e-8, seed=3, gates=None, **kw):
     m = AdaptiveMPS(n, bond_dim=1, eps_bond=eps, chi_max_per_bond=chi, device=DEV,
                     svd_method=method, **kw)
@@ -134,14 +150,9 @@ def _first_layer(seed, chi=256, sites=9, tau=10.0, angle=0.1):
 
 
 def _ranks_agree(got, want):
-    """The certified rank is the exact rule's rank, or within MAX_SPREAD of it
-    when the threshold lies within one float32 ulp of the total of the
-    partial sums."""
-    from atlas_q.rsvd import MAX_SPREAD
-    got, want = np.asarray(got), np.asarray(want)
-    assert got.shape == want.shape
-    assert np.abs(got - want).max() <= MAX_SPREAD, (got, want)
-    assert (got == want).mean() >= 0.75, (got, want)
+    """Certified ranks are the float32 rule's ranks on the exact spectrum,
+    which the FP64 Gram method's ranks are too."""
+    assert list(got) == list(want), (got, want)
 
 
 def test_rsvd_certified_results_match_gram():
@@ -193,8 +204,9 @@ def test_bf16x3_theta_path_fidelity():
 
 @pytest.mark.parametrize("method", ["plain", "gram"])
 def test_concurrent_sublayer_matches_sequential(method):
-    """Gates of one sub-layer on four HIP streams commit the same cores,
-    ranks and statistics as the one-by-one order."""
+    """A sub-layer applied at once (plain: four HIP streams; gram: one batched
+    eigensolver call) commits the same ranks, statistics and state as the
+    one-by-one order."""
     from atlas_q.workloads import gate_schedule, slice_tensors
     ts = slice_tensors(10, 64, "decay", seed=5, tau=12.0)
     gates = gate_schedule(10, 9, seed=6, angle=None)          # layer 0 (5) + layer 1 (4)
@@ -211,6 +223,7 @@ def test_concurrent_sublayer_matches_sequential(method):
     assert con.statistics.logs["bond"] == seq.statistics.logs["bond"]
     for x, y in zip(con.tensors, seq.tensors):
         assert x.shape == y.shape
-        assert float((x - y).abs().max()) <= 1e-6 * max(1.0, float(y.abs().max()))
+    # the factor phases are a gauge choice of the eigensolver: compare the state
+    assert abs(1 - _slice_fidelity(con, seq)) < 1e-9
     with pytest.raises(ValueError):
         con.apply_two_site_gates([(0, g[0][1]), (1, g[1][1])])
