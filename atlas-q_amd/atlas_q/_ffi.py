@@ -61,6 +61,9 @@ SIGNATURES = {
     "aq_apply_1q": (c_int, [c_int, c_vp, c_vp, c_i64, c_i64, c_vp, c_vp]),
     "aq_cgemm": (c_int, [c_int, c_int, c_i64, c_i64, c_i64, c_vp, c_i64, c_i64, c_int,
                          c_vp, c_i64, c_i64, c_int, c_vp, c_i64, c_i64, c_vp]),
+    "aq_cgemm_splitk": (c_int, [c_int, c_int, c_i64, c_i64, c_i64, c_vp, c_i64, c_i64, c_int,
+                                c_vp, c_i64, c_i64, c_int, c_vp, c_i64, c_i64, c_i64, c_vp, c_i64,
+                                c_vp]),
     "aq_sample_workspace_bytes": (c_i64, [c_int, c_i64, c_vp, c_i64]),
     "aq_sample": (c_int, [c_int, c_int, c_int, c_i64, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp,
                           c_i64, c_vp]),

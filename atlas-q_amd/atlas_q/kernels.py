@@ -158,10 +158,33 @@ def cgemm(A: torch.Tensor, B: torch.Tensor, conj_a: bool = False, conj_b: bool =
     N = B.shape[1]
     if out is None:
         out = torch.empty((M, N), dtype=A.dtype, device=A.device)
+    splits = splitk_for(M, N, K)
+    if splits > 1:
+        ws = torch.empty(splits * M * N, dtype=A.dtype, device=A.device)
+        check(lib().aq_cgemm_splitk(_code(A.dtype), PREC[prec], M, N, K, _ptr(A), A.stride(0),
+                                    A.stride(1), int(conj_a), _ptr(B), B.stride(0), B.stride(1),
+                                    int(conj_b), _ptr(out), out.stride(0), out.stride(1), splits,
+                                    _ptr(ws), ws.numel() * ws.element_size(), _stream(A)),
+              "aq_cgemm_splitk")
+        return out
     check(lib().aq_cgemm(_code(A.dtype), PREC[prec], M, N, K, _ptr(A), A.stride(0), A.stride(1),
                          int(conj_a), _ptr(B), B.stride(0), B.stride(1), int(conj_b), _ptr(out),
                          out.stride(0), out.stride(1), _stream(A)), "aq_cgemm")
     return out
+
+
+SPLITK_TILES = 128     # output tiles (64 x 128) below which the reduction is split
+SPLITK_TARGET = 512    # work-groups aimed at (two per CU)
+SPLITK_MIN_CHUNK = 64  # reduction length per slice
+
+
+def splitk_for(M: int, N: int, K: int) -> int:
+    """Reduction slices for an M x N x K product: 1 unless the output tiles
+    cannot fill the GPU and the reduction is long."""
+    tiles = -(-M // 64) * -(-N // 128)
+    if tiles >= SPLITK_TILES or K < 2 * SPLITK_MIN_CHUNK:
+        return 1
+    return int(max(1, min(K // SPLITK_MIN_CHUNK, max(2, SPLITK_TARGET // tiles))))
 
 
 # ---------------------------------------------------------------- sampler

@@ -138,6 +138,107 @@ __global__ __launch_bounds__(kThreads) void cgemm_bf16x3_kernel(AStrided<float> 
   });
 }
 
+// Split-K: work-group (x, y, z) computes the tile's partial product over the
+// reduction slice [z*chunk, (z+1)*chunk) into part[z][M][N]; the reduce kernel
+// adds the slices in order (deterministic) into C.
+template <typename T, bool KFastA, bool NFastB, bool kBf16x3>
+__global__ __launch_bounds__(kThreads) void cgemm_splitk_kernel(AStrided<T> aop, BStrided<T> bop,
+                                                                c2_t<T>* __restrict__ part,
+                                                                int64_t M, int64_t N, int64_t K,
+                                                                int64_t chunk) {
+  const int64_t m0 = (int64_t)blockIdx.x * kBM;
+  const int64_t j0 = (int64_t)blockIdx.y * kBJ;
+  const int64_t k0 = (int64_t)blockIdx.z * chunk;
+  const int64_t klen = (K - k0) < chunk ? (K - k0) : chunk;
+  AStrided<T> a = aop;
+  a.p += k0 * aop.s1;
+  a.K = klen;
+  BStrided<T> b = bop;
+  b.p += k0 * bop.s0;
+  b.K = klen;
+  ColMap cm{0, N};
+  Frag<T> fr;
+  if constexpr (kBf16x3) {
+    __shared__ SmemBf16 sm;
+    cgemm_mainloop_bf16x3<KFastA, NFastB>(a, b, cm, klen, m0, j0, sm, fr);
+  } else {
+    __shared__ Smem<T> sm;
+    cgemm_mainloop<T, KFastA, NFastB>(a, b, cm, klen, m0, j0, sm, fr);
+  }
+  c2_t<T>* out = part + (int64_t)blockIdx.z * M * N;
+  frag_foreach<T>(fr, [&](int row, int h, int jj, T re, T im) {
+    const int64_t m = m0 + row;
+    const int64_t n = 2 * j0 + h * kBJ + jj;
+    if (m < M && n < N) {
+      c2_t<T> v;
+      v.x = re;
+      v.y = im;
+      out[m * N + n] = v;
+    }
+  });
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void splitk_reduce_kernel(const c2_t<T>* __restrict__ part,
+                                                            c2_t<T>* __restrict__ C, int64_t M,
+                                                            int64_t N, int64_t splits,
+                                                            int64_t c_s0, int64_t c_s1) {
+  const int64_t total = M * N;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    T re = 0, im = 0;
+    for (int64_t z = 0; z < splits; ++z) {
+      const c2_t<T> v = part[z * total + e];
+      re += v.x;
+      im += v.y;
+    }
+    c2_t<T> o;
+    o.x = re;
+    o.y = im;
+    C[(e / N) * c_s0 + (e % N) * c_s1] = o;
+  }
+}
+
+template <typename T, bool KA, bool NB>
+static void launch_splitk(dim3 grid, hipStream_t s, AStrided<T> a, BStrided<T> b, void* part,
+                          int64_t M, int64_t N, int64_t K, int64_t chunk, int prec) {
+  if constexpr (std::is_same<T, float>::value) {
+    if (prec == AQ_PREC_BF16X3) {
+      cgemm_splitk_kernel<float, KA, NB, true><<<grid, kThreads, 0, s>>>(
+          a, b, (float2*)part, M, N, K, chunk);
+      return;
+    }
+  }
+  cgemm_splitk_kernel<T, KA, NB, false><<<grid, kThreads, 0, s>>>(a, b, (c2_t<T>*)part, M, N,
+                                                                   K, chunk);
+}
+
+template <typename T>
+static int cgemm_splitk_impl(int prec, int64_t M, int64_t N, int64_t K, const void* A,
+                             int64_t as0, int64_t as1, int ca, const void* B, int64_t bs0,
+                             int64_t bs1, int cb, void* C, int64_t cs0, int64_t cs1,
+                             int64_t splits, void* ws, int64_t ws_bytes, hipStream_t s) {
+  const int64_t chunk = cdiv(cdiv(K, splits), 16) * 16;
+  splits = cdiv(K, chunk);
+  AQ_REQUIRE(ws_bytes >= splits * M * N * (int64_t)sizeof(c2_t<T>), AQ_ERR_ARG,
+             "aq_cgemm_splitk: workspace of %lld bytes < %lld needed", (long long)ws_bytes,
+             (long long)(splits * M * N * (int64_t)sizeof(c2_t<T>)));
+  AStrided<T> a{(const c2_t<T>*)A, as0, as1, M, K, ca};
+  BStrided<T> b{(const c2_t<T>*)B, bs0, bs1, K, N, cb};
+  dim3 grid((unsigned)cdiv(M, kBM), (unsigned)cdiv(N, kBN), (unsigned)splits);
+  const bool ka = (as1 == 1), nb = (bs1 == 1);
+  if (ka && nb) launch_splitk<T, true, true>(grid, s, a, b, ws, M, N, K, chunk, prec);
+  else if (ka) launch_splitk<T, true, false>(grid, s, a, b, ws, M, N, K, chunk, prec);
+  else if (nb) launch_splitk<T, false, true>(grid, s, a, b, ws, M, N, K, chunk, prec);
+  else launch_splitk<T, false, false>(grid, s, a, b, ws, M, N, K, chunk, prec);
+  AQ_LAUNCH_CHECK();
+  const int64_t blocks = cdiv(M * N, 256) < 4096 ? cdiv(M * N, 256) : 4096;
+  splitk_reduce_kernel<T><<<(unsigned)blocks, 256, 0, s>>>((const c2_t<T>*)ws, (c2_t<T>*)C, M,
+                                                            N, splits, cs0, cs1);
+  AQ_LAUNCH_CHECK();
+  return AQ_OK;
+}
+
 template <typename T>
 static Gate4<T> make_gate(const double* g) {
   Gate4<T> out;
@@ -229,4 +330,23 @@ extern "C" int aq_cgemm(int dtype, int prec, int64_t M, int64_t N, int64_t K, co
   AQ_REQUIRE(prec == AQ_PREC_NATIVE, AQ_ERR_ARG, "aq_cgemm: complex128 is FP64 only");
   return cgemm_impl<double>(prec, M, N, K, A, a_s0, a_s1, conj_a, B, b_s0, b_s1, conj_b, C,
                             c_s0, c_s1, stream);
+}
+
+extern "C" int aq_cgemm_splitk(int dtype, int prec, int64_t M, int64_t N, int64_t K,
+                               const void* A, int64_t a_s0, int64_t a_s1, int conj_a,
+                               const void* B, int64_t b_s0, int64_t b_s1, int conj_b, void* C,
+                               int64_t c_s0, int64_t c_s1, int64_t splits, void* workspace,
+                               int64_t workspace_bytes, hipStream_t stream) {
+  AQ_REQUIRE(A && B && C && workspace, AQ_ERR_ARG, "aq_cgemm_splitk: null pointer");
+  AQ_REQUIRE(M > 0 && N > 0 && K > 0, AQ_ERR_ARG, "aq_cgemm_splitk: sizes must be positive");
+  AQ_REQUIRE(splits >= 1 && splits <= 1024, AQ_ERR_ARG, "aq_cgemm_splitk: splits in [1, 1024]");
+  AQ_REQUIRE(cdiv(M, kBM) < (1LL << 31) && cdiv(N, kBN) < 65536, AQ_ERR_ARG,
+             "aq_cgemm_splitk: grid too large");
+  if (dtype == AQ_C64)
+    return cgemm_splitk_impl<float>(prec, M, N, K, A, a_s0, a_s1, conj_a, B, b_s0, b_s1, conj_b,
+                                    C, c_s0, c_s1, splits, workspace, workspace_bytes, stream);
+  AQ_REQUIRE(dtype == AQ_C128, AQ_ERR_ARG, "aq_cgemm_splitk: unknown dtype %d", dtype);
+  AQ_REQUIRE(prec == AQ_PREC_NATIVE, AQ_ERR_ARG, "aq_cgemm_splitk: complex128 is FP64 only");
+  return cgemm_splitk_impl<double>(prec, M, N, K, A, a_s0, a_s1, conj_a, B, b_s0, b_s1, conj_b,
+                                   C, c_s0, c_s1, splits, workspace, workspace_bytes, stream);
 }

@@ -62,6 +62,12 @@ def parse():
     ap.add_argument("--no-low", action="store_true",
                     help="skip the low-entanglement (learned-width) measurement")
     ap.add_argument("--seed", type=int, default=2025)
+    ap.add_argument("--gemm-probe", action="store_true",
+                    help="only run 5 FP64 Gram GEMMs X X^H of the bench's 2048 x 2048 theta "
+                         "(the dominant hand-written kernel; for counter runs) and exit")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run 5 device-to-device copies of MIB MiB (calibrates the "
+                         "FETCH_SIZE / WRITE_SIZE counters for the traffic figure) and exit")
     return ap.parse_args()
 
 
@@ -189,6 +195,19 @@ def low_entanglement(torch, a, dev):
     return out
 
 
+def traffic_from_profile():
+    """HBM bytes per launch of the FP64 Gram GEMM at this shape, from the
+    committed counter summary (FETCH_SIZE and WRITE_SIZE runs, calibrated on a
+    copy of known size); None if the summary is absent."""
+    path = ROOT / "profiles" / "gram_fp64_cgemm_chi1024_traffic.md"
+    if not path.exists():
+        return None, None
+    for line in path.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch"):
+            return float(line.split("=")[1]), str(path.relative_to(ROOT))
+    return None, None
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -199,6 +218,24 @@ def main():
     import numpy as np
     import torch
     from atlas_q import kernels as K
+    if a.gemm_probe:
+        from atlas_q.adaptive_mps import _gate4
+        ts = slice_state(torch, 2, a.chi, a.seed, "cuda")
+        X = K.theta(ts[0], ts[1], _gate4(gate_schedule(2, 1, a.seed + 1)[0][1]))
+        X64 = X.to(torch.complex128)
+        for _ in range(5):
+            G = K.cgemm(X64, X64.T, conj_b=True)
+        torch.cuda.synchronize()
+        print(json.dumps({"gemm_probe": list(G.shape), "launches": 5}), flush=True)
+        return
+    if a.copy_probe:
+        src = torch.ones(a.copy_probe << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": src.numel() * 4, "copies": 5}), flush=True)
+        return
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
@@ -218,6 +255,8 @@ def main():
     RL.enable(False)
     ups = a.steps / t_fast
     roof = RL.summary()
+    if roof and roof["kernel"] == "gram_fp64_cgemm" and chi == 1024:
+        roof["traffic"], roof["traffic_source"] = traffic_from_profile()
     ks = fast.statistics.logs["k_star"][a.warmup:]
     fast_seq = None
     if a.workers > 1 and a.seq_steps > 0:

@@ -107,6 +107,18 @@ int aq_cgemm(int dtype, int prec, int64_t M, int64_t N, int64_t K,
              const void* B, int64_t b_s0, int64_t b_s1, int conj_b,
              void* C, int64_t c_s0, int64_t c_s1, hipStream_t stream);
 
+/* aq_cgemm for outputs too small to fill the GPU over a long reduction
+ * (sketches X Omega, Cholesky-QR and core Gram products Y^H Y of the
+ * truncated-SVD variants; no reference counterpart beyond the torch matmuls
+ * of the plain path).  The reduction is cut into `splits` slices computed by
+ * separate work-groups into `workspace` (>= splits*M*N complex elements of
+ * dtype) and added in slice order, so the result is deterministic. */
+int aq_cgemm_splitk(int dtype, int prec, int64_t M, int64_t N, int64_t K, const void* A,
+                    int64_t a_s0, int64_t a_s1, int conj_a, const void* B, int64_t b_s0,
+                    int64_t b_s1, int conj_b, void* C, int64_t c_s0, int64_t c_s1,
+                    int64_t splits, void* workspace, int64_t workspace_bytes,
+                    hipStream_t stream);
+
 /* Batched sweep sampler: replaces MatrixProductStatePyTorch._batch_sweep_sample
  * (src/atlas_q/mps_pytorch.py:247-355) with its multinomial draw expressed as
  * the inverse CDF  outcome = (u >= p0)  over caller-supplied uniforms.

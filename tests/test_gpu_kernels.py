@@ -82,6 +82,40 @@ def test_cgemm_matches_matmul(dtype, mnk, layout):
     assert _relerr(C, A64 @ B64) < 10 * TOL[dtype]
 
 
+@pytest.mark.parametrize("dtype", [torch.complex64, torch.complex128])
+@pytest.mark.parametrize("mnk", [(16, 16, 2048), (2048, 16, 2048), (250, 250, 2048),
+                                 (250, 2048, 1999), (5, 3, 700)])
+@pytest.mark.parametrize("layout", ["nn", "tn", "conj"])
+def test_cgemm_splitk_matches_matmul(dtype, mnk, layout):
+    """Skinny outputs over long reductions take the split-K path; the slices
+    are added in a fixed order, so two calls agree bit for bit."""
+    M, N, K_ = mnk
+    assert K.splitk_for(M, N, K_) > 1
+    g = torch.Generator().manual_seed(M * 7 + N + K_)
+    A = _rand((M, K_), dtype, g)
+    B = _rand((K_, N), dtype, g)
+    Ad, Bd = A.to(DEV), B.to(DEV)
+    ca = cb = layout == "conj"
+    if layout == "tn":
+        Ad = A.T.contiguous().to(DEV).T
+    C1 = K.cgemm(Ad, Bd, conj_a=ca, conj_b=cb)
+    C2 = K.cgemm(Ad, Bd, conj_a=ca, conj_b=cb)
+    A64, B64 = A.to(torch.complex128), B.to(torch.complex128)
+    if ca:
+        A64, B64 = A64.conj(), B64.conj()
+    assert _relerr(C1, A64 @ B64) < 10 * TOL[dtype]
+    assert torch.equal(C1, C2)
+
+
+def test_cgemm_splitk_bf16x3_error_bound():
+    g = torch.Generator().manual_seed(5)
+    A = _rand((2048, 2048), torch.complex64, g)
+    B = _rand((2048, 24), torch.complex64, g)
+    C = K.cgemm(A.to(DEV), B.to(DEV), prec="bf16x3")
+    ref = A.to(torch.complex128) @ B.to(torch.complex128)
+    assert float((C.cpu().to(torch.complex128) - ref).norm() / ref.norm()) < 2e-5
+
+
 def test_cgemm_bf16x3_error_bound():
     g = torch.Generator().manual_seed(3)
     A = _rand((256, 512), torch.complex64, g)
