@@ -22,24 +22,36 @@ import torch
 JACOBI_MAX = 64
 
 
-def heevd_batched(G: torch.Tensor):
+def heevd_batched(G: torch.Tensor, inplace: bool = False, return_ok: bool = False):
     """Eigenpairs of Hermitian complex128 matrices G [batch, n, n] (n > 64) in
     one rocSOLVER strided-batched call: (eigenvalues desc [batch, n],
-    eigenvectors as columns [batch, n, n]).  Raises if any matrix did not
-    converge; G is not modified."""
+    eigenvectors as columns [batch, n, n]).
+
+    G is left unmodified unless ``inplace`` (then G must be a contiguous
+    complex128 tensor and is overwritten by the eigenvectors).  Without
+    ``return_ok`` a matrix that did not converge raises; with it the call
+    returns a third value, a host list of per-matrix convergence flags, and
+    the caller handles the failed ones."""
     from . import kernels as K
     from ._ffi import check, lib
-    A = G.to(torch.complex128).contiguous().clone()
+    if inplace:
+        if G.dtype != torch.complex128 or not G.is_contiguous():
+            raise ValueError("heevd_batched(inplace=True) needs a contiguous complex128 batch")
+        A = G
+    else:
+        A = G.to(torch.complex128).contiguous().clone()
     bsz, n, _ = A.shape
     W = torch.empty((bsz, n), dtype=torch.float64, device=A.device)
     E = torch.empty((bsz, n), dtype=torch.float64, device=A.device)
     info = torch.zeros(bsz, dtype=torch.int32, device=A.device)
     check(lib().aq_heevd_batched(n, K._ptr(A), n * n, K._ptr(W), n, K._ptr(E), K._ptr(info),
                                  bsz, K._stream(A)), "aq_heevd_batched")
-    if int(info.abs().max()) != 0:
+    ok = [v == 0 for v in info.cpu().tolist()]
+    if not return_ok and not all(ok):
         raise RuntimeError(f"aq_heevd_batched: not converged (info {info.tolist()})")
     # row j of A holds conj(v_j): V = A^H, columns ascending -> flip to descending
-    return W.flip(-1), A.transpose(-2, -1).conj().flip(-1)
+    lam, V = W.flip(-1), A.transpose(-2, -1).conj().flip(-1)
+    return (lam, V, ok) if return_ok else (lam, V)
 
 
 def eigh_desc(G: torch.Tensor):

@@ -80,16 +80,17 @@ def default_predictor(device):
     return _DEFAULT[key]
 
 
-def _gram_matrix(X: torch.Tensor):
+def _gram_matrix(X: torch.Tensor, out: torch.Tensor = None):
     """FP64 Gram matrix of the smaller side of X: (G, left, X64) with left
-    True for G = X X^H [m, m], False for G = X^H X [n, n]."""
+    True for G = X X^H [m, m], False for G = X^H X [n, n]; written into
+    ``out`` (contiguous complex128 of that order) when given."""
     m, n = X.shape
     X64 = X.to(torch.complex128)
     if m <= n:
         with RL.timed("gram_fp64_cgemm", 8.0 * m * m * n, "fp64_mfma"):
-            return K.cgemm(X64, X64.T, conj_b=True), True, X64
+            return K.cgemm(X64, X64.T, conj_b=True, out=out), True, X64
     with RL.timed("gram_fp64_cgemm", 8.0 * n * n * m, "fp64_mfma"):
-        return K.cgemm(X64.T, X64, conj_a=True), False, X64
+        return K.cgemm(X64.T, X64, conj_a=True, out=out), False, X64
 
 
 def _gram_factors(X: torch.Tensor):
@@ -200,30 +201,41 @@ def _harvest(mps, feats, k):
 def finish_gram(mps, jobs):
     """Exact Gram factorization of several updates.  Gram matrices of equal
     order are factored together: orders <= 64 in the batched Jacobi kernel,
-    larger ones in one strided-batched rocSOLVER call per order."""
+    larger ones in one strided-batched rocSOLVER call per order, the Gram
+    GEMMs writing straight into the batch the eigensolver overwrites.  A
+    matrix the batched call leaves unconverged is rebuilt and retried alone
+    (``eigh_desc``: jittered retry), then falls back to the plain SVD."""
     from .eig import JACOBI_MAX, heevd_batched
     st = _factor_stats(mps)
-    grams = [_gram_matrix(j.X) for j in jobs]
-    eig = [None] * len(jobs)
     orders = {}
-    for idx, (G, _, _) in enumerate(grams):
-        orders.setdefault(G.shape[0], []).append(idx)
+    for idx, j in enumerate(jobs):
+        orders.setdefault(min(j.X.shape), []).append(idx)
+    grams = [None] * len(jobs)       # (G or None, left, X64)
+    eig = [None] * len(jobs)
     for n, idxs in orders.items():
-        try:
-            if n <= JACOBI_MAX or len(idxs) == 1:
-                for idx in idxs:
+        if n <= JACOBI_MAX or len(idxs) == 1:
+            for idx in idxs:
+                grams[idx] = _gram_matrix(jobs[idx].X)
+                try:
                     eig[idx] = eigh_desc(grams[idx][0])
-            else:
-                lam, V = heevd_batched(torch.stack([grams[idx][0] for idx in idxs]))
-                for b, idx in enumerate(idxs):
-                    eig[idx] = (lam[b], V[b])
-        except RuntimeError:
-            pass                        # per-matrix retry below
+                except RuntimeError:
+                    pass
+            continue
+        Gb = torch.empty((len(idxs), n, n), dtype=torch.complex128, device=jobs[idxs[0]].X.device)
+        for b, idx in enumerate(idxs):
+            _, left, X64 = _gram_matrix(jobs[idx].X, out=Gb[b])
+            grams[idx] = (None, left, X64)       # Gb[b] is overwritten below
+        lam, V, ok = heevd_batched(Gb, inplace=True, return_ok=True)
+        for b, idx in enumerate(idxs):
+            if ok[b]:
+                eig[idx] = (lam[b], V[b])
     outs = []
     for idx, j in enumerate(jobs):
         G, left, X64 = grams[idx]
         if eig[idx] is None:
             try:
+                if G is None:
+                    G = _gram_matrix(j.X)[0]
                 eig[idx] = eigh_desc(G)
             except RuntimeError:
                 # the FP64 eigensolver failed twice: the plain method's SVD cascade

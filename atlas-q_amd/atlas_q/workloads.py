@@ -30,17 +30,24 @@ def schmidt_profile(chi: int, kind: str = "flat", tau: float = 24.0) -> np.ndarr
 
 
 def random_right_isometry(chi_l: int, chi_r: int, gen: torch.Generator) -> torch.Tensor:
-    """G [chi_l, 2, chi_r] with orthonormal rows over (s, r) (needs chi_l <= 2 chi_r)."""
-    Z = torch.complex(torch.randn(2 * chi_r, chi_l, generator=gen, dtype=torch.float64),
-                      torch.randn(2 * chi_r, chi_l, generator=gen, dtype=torch.float64))
+    """G [chi_l, 2, chi_r] with orthonormal rows over (s, r) (needs chi_l <= 2 chi_r),
+    drawn and factored on the generator's device."""
+    dev = gen.device
+    Z = torch.complex(torch.randn(2 * chi_r, chi_l, generator=gen, dtype=torch.float64, device=dev),
+                      torch.randn(2 * chi_r, chi_l, generator=gen, dtype=torch.float64, device=dev))
     Q, _ = torch.linalg.qr(Z)
     return Q.T.conj().reshape(chi_l, 2, chi_r)
 
 
 def slice_tensors(sites: int, chi: int, kind: str = "flat", seed: int = 0, tau: float = 24.0,
-                  device="cuda", dtype=torch.complex64):
-    gen = torch.Generator().manual_seed(seed)
-    lam = torch.from_numpy(schmidt_profile(chi, kind, tau)).to(torch.complex128)
+                  device="cuda", dtype=torch.complex64, draw_on_device: bool = False):
+    """``sites`` tensors Lambda G [chi, 2, chi].  The random isometries are
+    drawn from a CPU generator (reproducible across machines) unless
+    ``draw_on_device``, which draws and factors them on ``device`` (fast for
+    long slices at large chi; reproducible on one device type)."""
+    gdev = torch.device(device) if draw_on_device else torch.device("cpu")
+    gen = torch.Generator(device=gdev).manual_seed(seed)
+    lam = torch.from_numpy(schmidt_profile(chi, kind, tau)).to(torch.complex128).to(gdev)
     out = []
     for _ in range(sites):
         G = random_right_isometry(chi, chi, gen)

@@ -2,11 +2,12 @@
 """Two-site gate updates/s at chi = 1024 on one MI355X.
 
 Workload: the steady state of the split-precision brickwork circuit (128
-qubits, depth 60, Haar SU(4), chi_max = 1024, cutoff 1e-8, complex64): a slice
-of SITES consecutive sites whose bonds -- including the two open ends -- are
-all at the cap 1024 (sites Lambda G: random right-isometries G, Schmidt
-weights of a saturated random circuit), so every update contracts a
-2048 x 2048 theta and keeps 1024 singular triplets.  Gates are seeded Haar SU(4) in brickwork order over
+qubits, depth 60, Haar SU(4), chi_max = 1024, cutoff 1e-8, complex64): a chain
+of SITES = 128 sites whose bonds -- including the two open ends -- are all at
+the cap 1024 (sites Lambda G: random right-isometries G, Schmidt weights of a
+saturated random circuit), so every update contracts a 2048 x 2048 theta and
+keeps 1024 singular triplets, and a brickwork sub-layer holds 64 (63) gates as
+in the circuit itself.  Gates are seeded Haar SU(4) in brickwork order over
 the slice's interior bonds.  A step is one two-site update
 (theta -> truncated SVD -> rank rule -> rebuild), tensors resident in HBM.
 The gates of one brickwork sub-layer act on disjoint bonds; both paths run
@@ -24,6 +25,10 @@ roofline     : the FP64 Gram GEMM (the hand-written kernel that dominates the
                path's own time), achieved FP64 TFLOP/s from HIP events
 cpu_baseline : the CPU oracle (torch-CPU restatement of the reference update)
                on the host cores, a bounded sample of updates at the same shape
+sampler      : shots/s of the fused sweep sampler on the plain-GPU circuit's
+               shape (64 qubits, chi_max 512, 10,000 shots, complex64, a random
+               right-canonical MPS at the circuit's bond profile), its FP32-MFMA
+               roofline and the CPU oracle's shots/s on a bounded sample
 """
 from __future__ import annotations
 
@@ -39,15 +44,24 @@ sys.path.insert(0, str(ROOT))
 sys.path.insert(0, str(ROOT / "atlas-q_amd"))
 
 METRIC = "2-site gate updates/s @chi=1024, learned-rank vs plain SVD, at equal fidelity"
+_T0 = time.perf_counter()
+
+
+def log(msg: str) -> None:
+    """Progress on stderr (the JSON line is the only stdout output)."""
+    print(f"[bench {time.perf_counter() - _T0:7.1f}s] {msg}", file=sys.stderr, flush=True)
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=18)
-    ap.add_argument("--warmup", type=int, default=9)
+    ap.add_argument("--steps", type=int, default=127,
+                    help="timed updates (127 = one odd and one even sub-layer of 128 sites)")
+    ap.add_argument("--warmup", type=int, default=64)
     ap.add_argument("--chi", type=int, default=1024)
-    ap.add_argument("--sites", type=int, default=10)
+    ap.add_argument("--sites", type=int, default=128)
+    ap.add_argument("--low-sites", type=int, default=10,
+                    help="slice length of the low-entanglement measurement")
     ap.add_argument("--workers", type=int, default=4,
                     help="host threads / HIP streams for the independent gates of a sub-layer")
     ap.add_argument("--method", default="learned")
@@ -61,6 +75,12 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-low", action="store_true",
                     help="skip the low-entanglement (learned-width) measurement")
+    ap.add_argument("--sampler-qubits", type=int, default=64)
+    ap.add_argument("--sampler-chi", type=int, default=512)
+    ap.add_argument("--sampler-shots", type=int, default=10000)
+    ap.add_argument("--sampler-reps", type=int, default=5)
+    ap.add_argument("--sampler-cpu-shots", type=int, default=128)
+    ap.add_argument("--no-sampler", action="store_true")
     ap.add_argument("--seed", type=int, default=2025)
     ap.add_argument("--gemm-probe", action="store_true",
                     help="only run 5 FP64 Gram GEMMs X X^H of the bench's 2048 x 2048 theta "
@@ -75,7 +95,7 @@ def slice_state(torch, sites, chi, seed, device):
     """Steady-state slice: Lambda G sites, G random right-isometries, Lambda the
     Schmidt weights of a saturated random circuit (all bonds at the cap)."""
     from atlas_q.workloads import slice_tensors
-    return slice_tensors(sites, chi, "flat", seed=seed, device=device)
+    return slice_tensors(sites, chi, "flat", seed=seed, device=device, draw_on_device=True)
 
 
 def gate_schedule(sites, count, seed):
@@ -162,7 +182,7 @@ def low_entanglement(torch, a, dev):
     from atlas_q.adaptive_mps import _gate4
     from atlas_q.workloads import gate_schedule as weak_schedule
     from atlas_q.workloads import slice_tensors
-    chi, sites = a.chi, a.sites
+    chi, sites = a.chi, a.low_sites
     ts = slice_tensors(sites, chi, "decay", seed=a.seed + 7, tau=10.0, device=dev)
     gates = weak_schedule(sites, (sites - 1 + 1) // 2, seed=a.seed + 8, angle=0.1)
     out = {"workload": f"first layer of weak gates exp(-0.1iH) on a {sites}-site slice at bond "
@@ -192,6 +212,67 @@ def low_entanglement(torch, a, dev):
     f = fidelity_vs_fp64(torch, ts[b], ts[b + 1], _gate4(U), chi, outs)
     out["fidelity"] = f
     out["abs_dF_learned_vs_plain"] = abs(f["fast"]["F"] - f["plain"]["F"])
+    return out
+
+
+def sampler_bench(torch, a, dev):
+    """Fused sweep sampler (one aq_sample call: per site the MFMA left-state
+    GEMM and the draw kernel) at the plain-GPU circuit's shape: n qubits,
+    bonds min(chi_max, 2^(b+1), 2^(n-b-1)), complex64, a random
+    right-canonical MPS (site 0 a unit vector, every other site a random
+    right-isometry) so that the sweep samples the Born distribution exactly."""
+    import numpy as np
+    from atlas_q import kernels as K
+    from atlas_q.workloads import random_right_isometry
+    n, cmax, S = a.sampler_qubits, a.sampler_chi, a.sampler_shots
+    chi = [1] + [min(cmax, 2 ** min(b + 1, n - b - 1)) for b in range(n - 1)] + [1]
+    gen = torch.Generator(device=dev).manual_seed(a.seed + 11)
+    ts = [random_right_isometry(chi[i], chi[i + 1], gen).to(torch.complex64).contiguous()
+          for i in range(n)]
+    u = torch.rand((S, n), dtype=torch.float64, device=dev, generator=gen)
+    K.sample_bits(ts, u)                                            # warm-up
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record()
+    for _ in range(a.sampler_reps):
+        bits, _ = K.sample_bits(ts, u)
+    e1.record()
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / a.sampler_reps
+    ms = e0.elapsed_time(e1) / a.sampler_reps
+    flops = float(S) * sum(16.0 * chi[i] * chi[i + 1] for i in range(n))
+    # tensors once, per shot and site: left state read (8 chiL), both outcome
+    # halves written (16 chiR), the uniform (8)
+    nbytes = sum(16.0 * chi[i] * chi[i + 1] for i in range(n)) + \
+        float(S) * sum(8.0 * chi[i] + 16.0 * chi[i + 1] + 8.0 for i in range(n))
+    tf = flops / (ms * 1e-3) / 1e12
+    gbs = nbytes / (ms * 1e-3) / 1e9
+    out = {"workload": f"{n} qubits, bonds min({cmax}, 2^(b+1), 2^(n-b-1)), {S} shots, "
+                       f"complex64, random right-canonical MPS",
+           "shots_per_s": S / wall, "ms_per_call": 1e3 * wall, "kernel_ms_per_call": ms,
+           "roofline": {"bound": "mfma", "achieved": tf, "peak": 157.3, "unit": "TFLOP/s",
+                        "frac": tf / 157.3, "flops_per_call": flops,
+                        "hbm_GBps": gbs, "hbm_frac": gbs / 8000.0}}
+    if not a.no_cpu and a.sampler_cpu_shots > 0:
+        from oracle import ref_mps as R
+        cores = len(os.sched_getaffinity(0))
+        torch.set_num_threads(cores)
+        cs = min(a.sampler_cpu_shots, S)
+        tc = [t.cpu() for t in ts]
+        uc = u[:cs].cpu().numpy()
+        t0 = time.perf_counter()
+        exp, p0 = R.batch_sweep_sample(tc, uc, return_p0=True)
+        t_cpu = time.perf_counter() - t0
+        got = np.array(K.bits_to_ints(bits[:cs]))
+        clear = (np.abs(uc - p0) > 1e-5).all(axis=1)
+        out["cpu_baseline"] = {"value": cs / t_cpu, "unit": "shots/s", "cores": cores,
+                               "kind": "port",
+                               "sample": f"{cs} of the {S} shots on the same MPS, torch CPU "
+                                         f"with {cores} threads"}
+        out["bit_exact_vs_oracle"] = {"shots": int(clear.sum()),
+                                      "equal": int((got[clear] == np.array(exp)[clear]).sum()),
+                                      "margin": 1e-5}
     return out
 
 
@@ -245,6 +326,8 @@ def main():
     sched = gate_schedule(sites, total + max(a.plain_steps, a.fidelity_samples, a.seq_steps),
                           a.seed + 1)
 
+    log(f"slice ready: {sites} sites at bond {chi}")
+
     # ---------------------------------------------------------------- fast path
     from atlas_q import roofline as RL
     fast = make_mps(torch, base, chi, a.method, a.theta_precision)
@@ -254,6 +337,7 @@ def main():
     t_fast = run_gates(torch, fast, sched[a.warmup: total], a.workers)
     RL.enable(False)
     ups = a.steps / t_fast
+    log(f"fast path: {ups:.2f} updates/s")
     roof = RL.summary()
     if roof and roof["kernel"] == "gram_fp64_cgemm" and chi == 1024:
         roof["traffic"], roof["traffic_source"] = traffic_from_profile()
@@ -276,6 +360,7 @@ def main():
             run_gates(torch, ps, sched[total - 1: total], 1)
             plain_seq = a.seq_steps / run_gates(torch, ps, sched[total: total + a.seq_steps], 1)
 
+    log(f"plain path: {plain} updates/s")
     # ---------------------------------------------------------------- fidelity
     fid = []
     if a.fidelity_samples > 0:
@@ -291,7 +376,11 @@ def main():
             fid.append(fidelity_vs_fp64(torch, state[b], state[b + 1], g, chi, outs))
     dF = max((abs(f["fast"]["F"] - f["plain"]["F"]) for f in fid), default=None)
 
+    log("fidelity done")
     low = None if a.no_low else low_entanglement(torch, a, dev)
+    log("low-entanglement done")
+    samp = None if a.no_sampler else sampler_bench(torch, a, dev)
+    log("sampler done")
 
     # ---------------------------------------------------------------- CPU baseline
     cpu = None
@@ -299,9 +388,10 @@ def main():
         from oracle import ref_mps as R
         cores = len(os.sched_getaffinity(0))
         torch.set_num_threads(cores)
-        om = R.OracleMPS(sites, bond_dim=1, eps_bond=1e-8, chi_max_per_bond=chi)
-        om.tensors = [t.cpu() for t in base]
-        om.bond_dims = [chi] * (sites - 1)
+        touched = 2 + max(b for b, _ in sched[: 1 + a.cpu_updates])
+        om = R.OracleMPS(touched, bond_dim=1, eps_bond=1e-8, chi_max_per_bond=chi)
+        om.tensors = [t.cpu() for t in base[:touched]]
+        om.bond_dims = [chi] * (touched - 1)
         b, U = sched[0]
         om.apply_two_site_gate(b, torch.from_numpy(U).to(torch.complex64))  # warm-up
         t0 = time.perf_counter()
@@ -336,6 +426,7 @@ def main():
         "low_entanglement": low,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "sampler": samp,
     }
     print(json.dumps(line), flush=True)
 
