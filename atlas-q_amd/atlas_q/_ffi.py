@@ -50,6 +50,23 @@ class RankResult(ctypes.Structure):
                 ("kept_weight", ctypes.c_double)]
 
 
+class TnTrunc(ctypes.Structure):
+    _fields_ = [("keep_count", ctypes.c_int64), ("chi", ctypes.c_int64),
+                ("adaptive_keep", ctypes.c_int64), ("prefix", ctypes.c_int32),
+                ("pad", ctypes.c_int32), ("total_weight", ctypes.c_double),
+                ("kept_weight", ctypes.c_double), ("truncation_error", ctypes.c_double)]
+
+
+class DenseLayer(ctypes.Structure):
+    _fields_ = [("W", ctypes.c_void_p), ("b", ctypes.c_void_p), ("ln_g", ctypes.c_void_p),
+                ("ln_b", ctypes.c_void_p), ("d_in", ctypes.c_int32), ("d_out", ctypes.c_int32),
+                ("act", ctypes.c_int32), ("pad", ctypes.c_int32), ("ln_eps", ctypes.c_double)]
+
+
+AQ_TN_POLICY = {"baseline": 0, "guardrail": 1, "blend": 2, "clamp": 3, "clamp+blend": 4}
+AQ_FEAT_LOG1P64, AQ_FEAT_FT135, AQ_FEAT_SPEC32 = 0, 1, 2
+AQ_ACT = {"none": 0, "relu": 1, "gelu": 2, "sigmoid": 3}
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "aq_version": (c_int, []),
@@ -86,6 +103,12 @@ SIGNATURES = {
     "aq_sample_grouped_site": (c_int, [c_vp, c_vp, c_vp, c_int, c_i64, c_i64, c_int, c_vp]),
     "aq_zero_u64": (c_int, [c_vp, c_i64, c_vp]),
     "aq_grouped_copy": (c_int, [c_vp, c_vp, c_int, c_i64, c_int, c_vp]),
+    "aq_tn_truncate": (c_int, [c_int, c_vp, c_i64, ctypes.c_double, c_int, c_i64, c_i64, c_int,
+                               ctypes.c_double, ctypes.c_double, c_vp, c_vp]),
+    "aq_rebuild_sqrt": (c_int, [c_int, c_vp, c_i64, c_i64, c_vp, c_vp, c_i64, c_i64, c_i64,
+                                c_i64, c_i64, c_vp, c_vp, c_vp]),
+    "aq_spectrum_features": (c_int, [c_int, c_int, c_vp, c_i64, c_i64, c_vp, c_vp, c_vp]),
+    "aq_dense_forward": (c_int, [c_int, c_vp, c_vp, c_i64, c_vp, c_vp]),
 }
 
 _lib = None
@@ -116,10 +139,11 @@ def lib() -> ctypes.CDLL:
 
 def check_layouts() -> None:
     """Assert that the NumPy descriptor layouts match the library's structs."""
-    out = (ctypes.c_int64 * 5)()
+    out = (ctypes.c_int64 * 7)()
     check(lib().aq_desc_sizes(out), "aq_desc_sizes")
     want = [GEMM_DESC.itemsize, SITE1Q_DESC.itemsize, SAMPLE_DESC.itemsize,
-            COPY_DESC.itemsize, ctypes.sizeof(RankResult)]
+            COPY_DESC.itemsize, ctypes.sizeof(RankResult), ctypes.sizeof(TnTrunc),
+            ctypes.sizeof(DenseLayer)]
     if list(out) != want:
         raise RuntimeError(f"descriptor layout mismatch: library {list(out)} vs binding {want}")
 

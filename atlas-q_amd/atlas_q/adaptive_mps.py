@@ -337,19 +337,34 @@ class AdaptiveMPS(MatrixProductStatePyTorch):
 
     # ------------------------------------------------------------ checkpoint
     def snapshot(self, path: str):
-        """adaptive_mps.py:387-404 (same dict; ``dtype`` added)."""
+        """adaptive_mps.py:387-404: the reference's dict (tensors, bond_dims,
+        num_qubits, eps_bond, chi_max_per_bond, statistics) plus two extension
+        keys the reference's loader ignores: ``dtype`` and ``dtype_policy``."""
         torch.save({"tensors": [t.cpu() for t in self.tensors], "bond_dims": self.bond_dims,
                     "num_qubits": self.num_qubits, "eps_bond": self.eps_bond,
                     "chi_max_per_bond": self.chi_max_per_bond,
-                    "statistics": self.statistics.logs, "dtype": str(self.dtype)}, path)
+                    "statistics": self.statistics.logs, "dtype": str(self.dtype),
+                    "dtype_policy": {"default": str(self.dtype_policy.default),
+                                     "promote_if_cond_gt":
+                                         float(self.dtype_policy.promote_if_cond_gt)}}, path)
 
     @staticmethod
     def load_snapshot(path: str, device: str = "cuda") -> "AdaptiveMPS":
-        """adaptive_mps.py:406-433."""
+        """adaptive_mps.py:406-433 (read with weights_only=True).  Snapshots
+        written by the reference carry no dtype policy and load with the
+        default one, as there; the extension keys restore it."""
         data = torch.load(path, weights_only=True)
+        kw = {}
+        pol = data.get("dtype_policy")
+        if isinstance(pol, dict) and "default" in pol:
+            dt = {"torch.complex64": torch.complex64,
+                  "torch.complex128": torch.complex128}.get(pol["default"])
+            if dt is not None:
+                kw["dtype_policy"] = DTypePolicy(default=dt, promote_if_cond_gt=float(
+                    pol.get("promote_if_cond_gt", 1e6)))
         mps = AdaptiveMPS(num_qubits=data["num_qubits"], bond_dim=max(data["bond_dims"]),
                           eps_bond=data["eps_bond"], chi_max_per_bond=data["chi_max_per_bond"],
-                          device=device)
+                          device=device, **kw)
         mps.tensors = [t.to(mps.device) for t in data["tensors"]]
         mps.bond_dims = data["bond_dims"]
         if "statistics" in data:

@@ -224,3 +224,76 @@ def sample_bits(tensors: Sequence[torch.Tensor], uniforms: torch.Tensor, prec: s
 def bits_to_ints(bits: torch.Tensor) -> List[int]:
     """The reference's sample ints: word 0 (low 64 bits) as signed int64."""
     return bits[:, 0].cpu().tolist()
+
+
+# ---------------------------------------------------------------- tn_core
+def tn_truncate(S: torch.Tensor, tol: float, adaptive: bool, chi_max, ai_keep: int,
+                policy: str, alpha: float, eps: float) -> "_ffi.TnTrunc":
+    """Truncation of tools_qih/tn_core.py:233-372 on the device; returns the
+    host copy of the result (one device->host transfer)."""
+    _dev(S, "S")
+    S = _plain(S).contiguous()
+    buf = torch.empty(ctypes.sizeof(_ffi.TnTrunc), dtype=torch.uint8, device=S.device)
+    pol = _ffi.AQ_TN_POLICY.get(policy, _ffi.AQ_TN_POLICY["guardrail"])
+    check(lib().aq_tn_truncate(_real_code(S.dtype), _ptr(S), S.numel(), float(tol),
+                               int(bool(adaptive)), -1 if chi_max is None else int(chi_max),
+                               int(ai_keep), pol, float(alpha), float(eps), _ptr(buf),
+                               _stream(S)), "aq_tn_truncate")
+    return _ffi.TnTrunc.from_buffer_copy(buf.cpu().numpy().tobytes())
+
+
+def rebuild_sqrt(U: torch.Tensor, S: torch.Tensor, Vh: torch.Tensor, k: int, chiL: int,
+                 chiR: int):
+    """A' = (U[:, :k] * sqrt(S[:k])).reshape(chiL, 2, k),
+    B' = (sqrt(S[:k])[:, None] * Vh[:k]).reshape(k, 2, chiR)."""
+    _dev(U, "U")
+    U, Vh = _plain(U), _plain(Vh)
+    dt = U.dtype
+    A = torch.empty((chiL, 2, k), dtype=dt, device=U.device)
+    B = torch.empty((k, 2, chiR), dtype=dt, device=U.device)
+    S = S.contiguous()
+    check(lib().aq_rebuild_sqrt(_code(dt), _ptr(U), U.stride(0), U.stride(1), _ptr(S), _ptr(Vh),
+                                Vh.stride(0), Vh.stride(1), 2 * chiL, 2 * chiR, int(k), _ptr(A),
+                                _ptr(B), _stream(U)), "aq_rebuild_sqrt")
+    return A, B
+
+
+def spectrum_features(S: torch.Tensor, kind: str, limit: int = -1):
+    """Predictor inputs from a spectrum on the device: kind "log1p64"
+    (RankPredictor, 64 floats over the first min(limit, 64) values; limit < 0
+    means #(S > 1e-10)) or "ft135" (FinetunedMLP, 135 floats).
+    Returns (features float32 [1, d], n_sv int64 [1]) on the device."""
+    _dev(S, "S")
+    S = _plain(S).contiguous()
+    code, d = {"log1p64": (_ffi.AQ_FEAT_LOG1P64, 64), "ft135": (_ffi.AQ_FEAT_FT135, 135),
+               "spec32": (_ffi.AQ_FEAT_SPEC32, 32)}[kind]
+    feat = torch.empty((1, d), dtype=torch.float32, device=S.device)
+    nsv = torch.empty(1, dtype=torch.int64, device=S.device)
+    check(lib().aq_spectrum_features(code, _real_code(S.dtype), _ptr(S), S.numel(), int(limit),
+                                     _ptr(feat), _ptr(nsv), _stream(S)), "aq_spectrum_features")
+    return feat, nsv
+
+
+def dense_forward(layers, x: torch.Tensor) -> torch.Tensor:
+    """FP32 dense stack; layers: list of dicts with device tensors W [d_out,
+    d_in], b, ln_g, ln_b (optional), act name and ln_eps."""
+    _dev(x, "x")
+    x = x.to(torch.float32).contiguous()
+    arr = (_ffi.DenseLayer * len(layers))()
+    for l, L in enumerate(layers):
+        W = L["W"]
+        assert W.dtype == torch.float32 and W.is_contiguous() and W.device == x.device
+        arr[l].W = W.data_ptr()
+        arr[l].b = L["b"].data_ptr() if L.get("b") is not None else None
+        arr[l].ln_g = L["ln_g"].data_ptr() if L.get("ln_g") is not None else None
+        arr[l].ln_b = L["ln_b"].data_ptr() if L.get("ln_b") is not None else None
+        arr[l].d_out, arr[l].d_in = int(W.shape[0]), int(W.shape[1])
+        arr[l].act = _ffi.AQ_ACT[L.get("act", "none")]
+        arr[l].ln_eps = float(L.get("ln_eps", 1e-5))
+    if x.shape[-1] != arr[0].d_in:
+        raise ValueError(f"dense_forward: input width {x.shape[-1]} != {arr[0].d_in}")
+    batch = x.numel() // arr[0].d_in
+    y = torch.empty((batch, arr[len(layers) - 1].d_out), dtype=torch.float32, device=x.device)
+    check(lib().aq_dense_forward(len(layers), ctypes.cast(arr, ctypes.c_void_p), _ptr(x), batch,
+                                 _ptr(y), _stream(x)), "aq_dense_forward")
+    return y

@@ -274,6 +274,8 @@ extern "C" int aq_desc_sizes(int64_t* out) {
   out[2] = (int64_t)sizeof(aq_sample_desc_t);
   out[3] = (int64_t)sizeof(aq_copy_desc_t);
   out[4] = (int64_t)sizeof(aq_rank_result_t);
+  out[5] = (int64_t)sizeof(aq_tn_trunc_t);
+  out[6] = (int64_t)sizeof(aq_dense_layer_t);
   return AQ_OK;
 }
 

@@ -81,6 +81,11 @@ def parse():
     ap.add_argument("--sampler-reps", type=int, default=5)
     ap.add_argument("--sampler-cpu-shots", type=int, default=128)
     ap.add_argument("--no-sampler", action="store_true")
+    ap.add_argument("--mark", action="store_true",
+                    help="launch a one-element zero-fill kernel right before and after the "
+                         "timed fast-path region (dispatch markers for trace filtering)")
+    ap.add_argument("--sampler-only", action="store_true",
+                    help="only run the sampler measurement and print its JSON object")
     ap.add_argument("--seed", type=int, default=2025)
     ap.add_argument("--gemm-probe", action="store_true",
                     help="only run 5 FP64 Gram GEMMs X X^H of the bench's 2048 x 2048 theta "
@@ -276,6 +281,17 @@ def sampler_bench(torch, a, dev):
     return out
 
 
+def marker(torch):
+    """One launch of the library's zero-fill kernel on one word: a dispatch
+    that brackets a region in a kernel trace."""
+    import ctypes
+    from atlas_q._ffi import lib
+    buf = torch.empty(1, dtype=torch.int64, device="cuda")
+    lib().aq_zero_u64(ctypes.c_void_p(buf.data_ptr()), 1,
+                      ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+    torch.cuda.synchronize()
+
+
 def traffic_from_profile():
     """HBM bytes per launch of the FP64 Gram GEMM at this shape, from the
     committed counter summary (FETCH_SIZE and WRITE_SIZE runs, calibrated on a
@@ -320,6 +336,9 @@ def main():
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
+    if a.sampler_only:
+        print(json.dumps({"sampler": sampler_bench(torch, a, dev)}), flush=True)
+        return
     chi, sites = a.chi, a.sites
     base = slice_state(torch, sites, chi, a.seed, dev)
     total = a.warmup + a.steps
@@ -334,7 +353,11 @@ def main():
     run_gates(torch, fast, sched[: a.warmup], a.workers)
     RL.reset()
     RL.enable(True)
+    if a.mark:
+        marker(torch)
     t_fast = run_gates(torch, fast, sched[a.warmup: total], a.workers)
+    if a.mark:
+        marker(torch)
     RL.enable(False)
     ups = a.steps / t_fast
     log(f"fast path: {ups:.2f} updates/s")

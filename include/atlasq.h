@@ -44,7 +44,8 @@ extern "C" {
 int aq_version(void);
 const char* aq_last_error(void);
 /* sizeof of aq_gemm_desc_t, aq_site1q_desc_t, aq_sample_desc_t, aq_copy_desc_t,
- * aq_rank_result_t (5 int64), so bindings can check their struct layouts. */
+ * aq_rank_result_t, aq_tn_trunc_t, aq_dense_layer_t (7 int64), so bindings
+ * can check their struct layouts. */
 int aq_desc_sizes(int64_t* out);
 
 /* Theta: X = gate-applied two-site contraction, "U" convention
@@ -261,6 +262,93 @@ typedef struct {
 } aq_copy_desc_t;
 int aq_grouped_copy(const aq_copy_desc_t* descs, const int64_t* prefix, int count,
                     int64_t total_elems, int grid, hipStream_t stream);
+
+/* ---------------------------------------------------------------- tn_core
+ * The reference's second two-site update, tools_qih/tn_core.py:mps_apply_2q
+ * (tn_core.py:186-384): theta in the U^T convention (aq_theta with the gate
+ * transposed), SVD, then the truncation below and the sqrt(S) split. */
+
+/* AI-policy codes (QIH_AI_POLICY, tn_core.py:282-333; unknown names map to
+ * guardrail on the host, as the reference's fallback branch does). */
+#define AQ_TN_POLICY_BASELINE 0
+#define AQ_TN_POLICY_GUARDRAIL 1
+#define AQ_TN_POLICY_BLEND 2
+#define AQ_TN_POLICY_CLAMP 3
+#define AQ_TN_POLICY_CLAMP_BLEND 4
+
+typedef struct {
+  int64_t keep_count;    /* singular values kept before the S > eps filter */
+  int64_t chi;           /* resulting bond dimension (#S[:keep_count] > eps) */
+  int64_t adaptive_keep; /* the adaptive cap (the SVD event's adaptive_keep) */
+  int32_t prefix;        /* 1 when the S > eps survivors are a prefix */
+  int32_t pad;
+  double total_weight;   /* sum S^2 */
+  double kept_weight;    /* sum over the kept values of S^2 */
+  double truncation_error; /* 1 - kept/total (0 when total == 0) */
+} aq_tn_trunc_t;
+
+/* Truncation of tn_core.py:233-372 on a descending spectrum S of m values
+ * (float32 for AQ_C64, float64 for AQ_C128), one result written to device
+ * memory `out`.  adaptive/tol: the adaptive rule keep = #(cumsum(S^2) <=
+ * (1-tol) total) + 1 (at most m; 1 when none), applied when adaptive and tol
+ * > 0.  chi_max < 0 means None.  ai_keep >= 0: the predictor's rank (already
+ * capped by chi_max) combined with the adaptive cap by `policy` (blend weight
+ * alpha); ai_keep < 0: no predictor (adaptive rule, else the fixed cap).
+ * Arithmetic follows torch's CPU rounding (cumsum accumulated in double,
+ * threshold float(1-tol) * total in S's type, S > eps with eps in S's type). */
+int aq_tn_truncate(int s_dtype, const void* S, int64_t m, double tol, int adaptive,
+                   int64_t chi_max, int64_t ai_keep, int policy, double alpha, double eps,
+                   aq_tn_trunc_t* out, hipStream_t stream);
+
+/* Core split of tn_core.py:374-382: A'[r, c] = U[r, c] * sqrt(S[c]) as
+ * [rows/2, 2, k], B'[r, c] = sqrt(S[r]) * Vh[r, c] as [k, 2, cols/2]; strides
+ * and types as aq_rebuild. */
+int aq_rebuild_sqrt(int dtype, const void* U, int64_t u_s0, int64_t u_s1, const void* S,
+                    const void* Vh, int64_t v_s0, int64_t v_s1, int64_t rows, int64_t cols,
+                    int64_t k, void* A_out, void* B_out, hipStream_t stream);
+
+/* Inputs of the reference's learned rank predictors from a spectrum S (m
+ * values, descending, float32/float64 by s_dtype), on the device.
+ * AQ_FEAT_LOG1P64: RankPredictor.forward's input (ai_rank_predictor.py:55-69):
+ *   64 floats log1p(S[i]/S[0]) for i < min(limit, m, 64), zero elsewhere; limit
+ *   is forward's max_ranks, or #(S > 1e-10) when negative (as predict passes).
+ * AQ_FEAT_FT135: FinetunedPredictorWrapper._features (finetuned_predictor.py:
+ *   51-75): 128 shifted log singular values, spectral entropy, cumulative mass
+ *   at 8, 16, 32, 64, 96, 128; 135 floats.
+ * n_sv (device int64): #(S > 1e-10), the rank both predictors scale by. */
+#define AQ_FEAT_LOG1P64 0
+#define AQ_FEAT_FT135 1
+#define AQ_FEAT_SPEC32 2 /* no reference counterpart: the inputs of this library's
+                            spectral predictor for the predict(S) seam of
+                            tn_core.py:294 -- with p_i = S_i^2 / sum S^2 over the
+                            leading <= 128 values: log10 p_i (i < 24), log10 of the
+                            weight beyond 8,16,32,64,96,128 values, entropy,
+                            log2 #(S > 1e-10); 32 floats */
+int aq_spectrum_features(int kind, int s_dtype, const void* S, int64_t m, int64_t limit,
+                         float* feat, int64_t* n_sv, hipStream_t stream);
+
+/* Forward pass of a stack of dense layers in FP32, the architectures of the
+ * reference's RankPredictor (ai_rank_predictor.py:24-42: Linear, LayerNorm,
+ * ReLU; Dropout is the identity at inference) and FinetunedMLP
+ * (finetuned_predictor.py:11-23: Linear, ReLU, Sigmoid).  Layer l computes
+ * z = W z_prev + b (W row-major [d_out, d_in]), then LayerNorm when ln_g is
+ * set (biased variance, eps ln_eps, affine ln_g / ln_b), then `act`.
+ * layers: host array of n_layers (1..8) descriptors of device pointers; dims
+ * 1..512.  x: [batch, d_in of layer 0], y: [batch, d_out of the last layer]. */
+#define AQ_ACT_NONE 0
+#define AQ_ACT_RELU 1
+#define AQ_ACT_GELU 2
+#define AQ_ACT_SIGMOID 3
+typedef struct {
+  const float* W;
+  const float* b;    /* nullable */
+  const float* ln_g; /* nullable: no LayerNorm */
+  const float* ln_b; /* nullable */
+  int32_t d_in, d_out, act, pad;
+  double ln_eps;
+} aq_dense_layer_t;
+int aq_dense_forward(int n_layers, const aq_dense_layer_t* layers, const float* x, int64_t batch,
+                     float* y, hipStream_t stream);
 
 #ifdef __cplusplus
 }

@@ -157,6 +157,22 @@ def test_snapshot_roundtrip(tmp_path):
     m2 = AdaptiveMPS.load_snapshot(str(p), device=DEV)
     assert m2.bond_dims == m.bond_dims
     assert torch.allclose(m2.to_statevector(), m.to_statevector())
+    # the dtype policy travels in the extension keys ...
+    m = AdaptiveMPS(5, bond_dim=1, device=DEV,
+                    dtype_policy=DTypePolicy(default=torch.complex128, promote_if_cond_gt=1e9))
+    m.snapshot(str(p))
+    m3 = AdaptiveMPS.load_snapshot(str(p), device=DEV)
+    assert m3.dtype_policy.default == torch.complex128
+    assert m3.dtype_policy.promote_if_cond_gt == 1e9
+    # ... and a snapshot in the reference's format (no extension keys) loads
+    # with the default policy
+    d = torch.load(str(p), weights_only=True)
+    d.pop("dtype")
+    d.pop("dtype_policy")
+    torch.save(d, str(p))
+    m4 = AdaptiveMPS.load_snapshot(str(p), device=DEV)
+    assert m4.dtype_policy.default == torch.complex64
+    assert m4.bond_dims == m.bond_dims
 
 
 # ---------------------------------------------------------------- sampler
