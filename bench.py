@@ -364,6 +364,16 @@ def main():
     roof = RL.summary()
     if roof and roof["kernel"] == "gram_fp64_cgemm" and chi == 1024:
         roof["traffic"], roof["traffic_source"] = traffic_from_profile()
+    eig = None
+    if roof and "eig_zheevd_batched" in roof["all"]:
+        e = roof["all"]["eig_zheevd_batched"]
+        eig = {"what": "rocSOLVER strided-batched zheevd over the FP64 Gram cores of a sub-layer, "
+                       "HIP events around each call; bytes = n^3/3 complex128 per matrix, the "
+                       "trailing blocks its one-stage tridiagonalisation streams",
+               "calls": e["launches"], "mean_ms_per_call": e["mean_ms"],
+               "ms_per_matrix": e["total_ms"] / a.steps,
+               "share_of_timed_region": e["total_ms"] / (1e3 * t_fast),
+               "streamed_GBps": e["achieved"], "frac_of_hbm_peak": e["frac"]}
     ks = fast.statistics.logs["k_star"][a.warmup:]
     fast_seq = None
     if a.workers > 1 and a.seq_steps > 0:
@@ -448,6 +458,7 @@ def main():
         "factor_rejects": fast.factor_stats.rejects if hasattr(fast, "factor_stats") else None,
         "low_entanglement": low,
         "roofline": roof,
+        "eigensolver": eig,
         "cpu_baseline": cpu,
         "sampler": samp,
     }
