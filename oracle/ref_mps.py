@@ -12,8 +12,8 @@ reference checkout).
 
 Parity pinning
 --------------
-* ``choose_rank_from_sigma``: hand-computed known answers + reference golden
-  vectors (``tests/golden/rank_rule.npz``).
+* ``choose_rank_from_sigma``: hand-computed known answers (tests/test_oracle.py)
+  + the per-gate kept ranks of the reference's golden runs.
 * ``theta``: closed-form einsum known answer (the reference's own
   ``tests/integration/test_mps_triton_kernels.py:34-57`` formula) + golden.
 * ``OracleMPS`` two-site update, canonical forms, statevector: golden vectors
