@@ -18,6 +18,7 @@ AQ_OK = 0
 AQ_C64, AQ_C128 = 0, 1
 AQ_PREC_NATIVE, AQ_PREC_BF16X3 = 0, 1
 AQ_RANK_REF = 0
+AQ_RANK_TDVP = 1
 
 c_i64 = ctypes.c_int64
 c_int = ctypes.c_int
@@ -109,6 +110,10 @@ SIGNATURES = {
                                 c_i64, c_i64, c_vp, c_vp, c_vp]),
     "aq_spectrum_features": (c_int, [c_int, c_int, c_vp, c_i64, c_i64, c_vp, c_vp, c_vp]),
     "aq_dense_forward": (c_int, [c_int, c_vp, c_vp, c_i64, c_vp, c_vp]),
+    "aq_mpo_apply_site": (c_int, [c_int, c_vp, c_i64, c_i64, c_vp, c_i64, c_i64, c_vp, c_vp]),
+    "aq_mpo_env_combine": (c_int, [c_int, c_vp, c_vp, c_i64, c_i64, c_i64, c_i64, c_vp, c_vp]),
+    "aq_fro_normalize_batched": (c_int, [c_int, c_vp, c_i64, c_i64, c_i64, ctypes.c_double,
+                                         c_vp]),
 }
 
 _lib = None

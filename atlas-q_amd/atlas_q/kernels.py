@@ -99,13 +99,14 @@ def theta(A: torch.Tensor, B: torch.Tensor, gate64: np.ndarray, prec: str = "nat
 _RANK_BYTES = ctypes.sizeof(RankResult)
 
 
-def rank_select_async(S: torch.Tensor, eps: float, cap: int) -> torch.Tensor:
+def rank_select_async(S: torch.Tensor, eps: float, cap: int, mode: int = _ffi.AQ_RANK_REF
+                      ) -> torch.Tensor:
     """Launch the rank rule; returns the device result buffer (uint8[48])."""
     _dev(S, "S")
     S = _plain(S).contiguous()
     buf = torch.empty(_RANK_BYTES, dtype=torch.uint8, device=S.device)
     check(lib().aq_rank_select(_real_code(S.dtype), _ptr(S), S.numel(), float(eps), int(cap),
-                               _ffi.AQ_RANK_REF, _ptr(buf), _stream(S)), "aq_rank_select")
+                               int(mode), _ptr(buf), _stream(S)), "aq_rank_select")
     return buf
 
 
@@ -114,8 +115,9 @@ def rank_result(buf: torch.Tensor) -> RankResult:
     return RankResult.from_buffer_copy(host.tobytes())
 
 
-def rank_select(S: torch.Tensor, eps: float, cap: int) -> RankResult:
-    return rank_result(rank_select_async(S, eps, cap))
+def rank_select(S: torch.Tensor, eps: float, cap: int, mode: int = _ffi.AQ_RANK_REF
+                ) -> RankResult:
+    return rank_result(rank_select_async(S, eps, cap, mode))
 
 
 # ---------------------------------------------------------------- rebuild
@@ -297,3 +299,83 @@ def dense_forward(layers, x: torch.Tensor) -> torch.Tensor:
     check(lib().aq_dense_forward(len(layers), ctypes.cast(arr, ctypes.c_void_p), _ptr(x), batch,
                                  _ptr(y), _stream(x)), "aq_dense_forward")
     return y
+
+
+# ---------------------------------------------------------------- MPO / envs
+def mpo_apply_site(W: torch.Tensor, A: torch.Tensor) -> torch.Tensor:
+    """M[(l a), s, (r b)] = sum_t W[l, s, t, r] A[a, t, b]."""
+    _dev(A, "A")
+    W = _plain(W).to(A.dtype).to(A.device).contiguous()
+    A = _plain(A).contiguous()
+    Dl, Dr = int(W.shape[0]), int(W.shape[3])
+    if tuple(W.shape[1:3]) != (2, 2) or A.shape[1] != 2:
+        raise ValueError(f"mpo_apply_site: W {tuple(W.shape)}, A {tuple(A.shape)}")
+    cl, cr = int(A.shape[0]), int(A.shape[2])
+    M = torch.empty((Dl * cl, 2, Dr * cr), dtype=A.dtype, device=A.device)
+    check(lib().aq_mpo_apply_site(_code(A.dtype), _ptr(W), Dl, Dr, _ptr(A), cl, cr, _ptr(M),
+                                  _stream(A)), "aq_mpo_apply_site")
+    return M
+
+
+def mpo_env_combine(Z: torch.Tensor, W: torch.Tensor) -> torch.Tensor:
+    """E'[R, r, B] = sum_{L,s,t} W[L,s,t,R] Z[L, t, r, s*chiB + B]."""
+    _dev(Z, "Z")
+    Z = _plain(Z).contiguous()
+    W = _plain(W).to(Z.dtype).to(Z.device).contiguous()
+    DL, _, chir, two_chiB = Z.shape
+    DR = int(W.shape[3])
+    chiB = two_chiB // 2
+    E = torch.empty((DR, chir, chiB), dtype=Z.dtype, device=Z.device)
+    check(lib().aq_mpo_env_combine(_code(Z.dtype), _ptr(Z), _ptr(W), int(DL), DR, int(chir),
+                                   int(chiB), _ptr(E), _stream(Z)), "aq_mpo_env_combine")
+    return E
+
+
+def fro_normalize_(X: torch.Tensor, floor: float = 1e-12) -> torch.Tensor:
+    """In place: each X[b] (leading index) divided by max(||X[b]||_F, floor)."""
+    _dev(X, "X")
+    assert X.is_contiguous() and not X.is_conj()
+    batch = int(X.shape[0])
+    count = X.numel() // max(batch, 1)
+    check(lib().aq_fro_normalize_batched(_code(X.dtype), _ptr(X), batch, count, count,
+                                         float(floor), _stream(X)), "aq_fro_normalize_batched")
+    return X
+
+
+def grouped_gemm(problems, dtype: torch.dtype, device) -> None:
+    """C_p = op(A_p) op(B_p) for a list of (A, B, C, conj_a, conj_b) 2-D
+    tensor views in one grouped launch (all of one dtype; every A shares its
+    fast axis, as every B does)."""
+    if not problems:
+        return
+    descs = np.zeros(len(problems), _ffi.GEMM_DESC)
+    ka = nb = None
+    for i, (A, B, C, ca, cb) in enumerate(problems):
+        assert not A.is_conj() and not B.is_conj() and C.dtype == dtype
+        d = descs[i]
+        d["A"], d["B"], d["C"] = A.data_ptr(), B.data_ptr(), C.data_ptr()
+        d["M"], d["K"] = A.shape
+        d["N"] = B.shape[1]
+        d["a_s0"], d["a_s1"] = A.stride()
+        d["b_s0"], d["b_s1"] = B.stride()
+        d["c_s0"], d["c_s1"] = C.stride()
+        d["conj_a"], d["conj_b"], d["kind"] = int(ca), int(cb), _ffi.AQ_GROUPED_GEMM
+        a_fast, b_fast = A.stride(1) == 1, B.stride(1) == 1
+        if ka is None:
+            ka, nb = a_fast, b_fast
+        elif (ka, nb) != (a_fast, b_fast):
+            raise ValueError("grouped_gemm: operands must share their layouts")
+    tiles = (-(-descs["M"] // 64)) * (-(-descs["N"] // 128))
+    pre = np.zeros(len(tiles), np.int64)
+    pre[1:] = np.cumsum(tiles[:-1])
+    total = int(tiles.sum())
+    dd = torch.from_numpy(descs.view(np.uint8).copy()).to(device)
+    pd = torch.from_numpy(pre).to(device)
+    grid = int(min(total, 65536))
+    check(lib().aq_grouped_gemm(_code(dtype), ctypes.c_void_p(dd.data_ptr()),
+                                ctypes.c_void_p(pd.data_ptr()), len(descs), total,
+                                int(ka) | (int(nb) << 1), grid,
+                                ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)),
+          "aq_grouped_gemm")
+    # keep the descriptor uploads alive until the launch has consumed them
+    torch.cuda.current_stream(device).synchronize()

@@ -284,8 +284,10 @@ extern "C" int aq_rank_select(int s_dtype, const void* S, int64_t m, double eps,
   AQ_REQUIRE(S && out, AQ_ERR_ARG, "aq_rank_select: null pointer");
   AQ_REQUIRE(m > 0, AQ_ERR_ARG, "aq_rank_select: empty spectrum");
   AQ_REQUIRE(cap >= 1, AQ_ERR_ARG, "aq_rank_select: cap must be >= 1");
-  AQ_REQUIRE(mode == AQ_RANK_REF, AQ_ERR_ARG, "aq_rank_select: unknown mode %d", mode);
-  const double c = 1.0 - eps * eps;  // Python-float (1 - eps**2), truncation.py:58
+  AQ_REQUIRE(mode == AQ_RANK_REF || mode == AQ_RANK_TDVP, AQ_ERR_ARG,
+             "aq_rank_select: unknown mode %d", mode);
+  // Python-float (1 - eps**2), truncation.py:58; (1 - eps), tdvp.py:553
+  const double c = mode == AQ_RANK_TDVP ? 1.0 - eps : 1.0 - eps * eps;
   if (s_dtype == AQ_C64) {
     rank_kernel<float><<<1, 256, 0, stream>>>((const float*)S, m, (float)c, cap, out);
   } else {

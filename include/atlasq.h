@@ -40,6 +40,9 @@ extern "C" {
 
 /* rank-rule arithmetic */
 #define AQ_RANK_REF 0 /* reference CPU semantics in S's dtype (see aq_rank_select) */
+#define AQ_RANK_TDVP 1 /* the same with threshold (1 - eps) * total instead of
+                          (1 - eps^2) * total: the split of TDVP2Site._evolve_two_site
+                          (src/atlas_q/tdvp.py:546-562) */
 
 int aq_version(void);
 const char* aq_last_error(void);
@@ -349,6 +352,29 @@ typedef struct {
 } aq_dense_layer_t;
 int aq_dense_forward(int n_layers, const aq_dense_layer_t* layers, const float* x, int64_t batch,
                      float* y, hipStream_t stream);
+
+/* ---------------------------------------------------------------- MPO / envs
+ * Callers next to the two-site path.  dtype AQ_C64 / AQ_C128, contiguous. */
+
+/* MPO x MPS site contraction of apply_mpo_to_mps (src/atlas_q/mpo_ops.py:
+ * 1034-1082): M[(l*chiL + a), s, (r*chiR + b)] = sum_t W[l,s,t,r] A[a,t,b],
+ * W [Dl, 2, 2, Dr], A [chiL, 2, chiR], M [Dl*chiL, 2, Dr*chiR]. */
+int aq_mpo_apply_site(int dtype, const void* W, int64_t Dl, int64_t Dr, const void* A,
+                      int64_t chiL, int64_t chiR, void* M, hipStream_t stream);
+
+/* Combine step of the MPO expectation-value transfer (mpo_ops.py:1083-1150,
+ * E' = einsum("Lab,atr,LstR,bsB->RrB", E, conj(A), W, A)): given
+ * Z[L, t] = conj(A[:, t, :])^T E[L] A  (Z [DL, 2, chir, 2*chiB], the two
+ * ket outcomes s side by side), E'[R, r, B] = sum_{L,s,t} W[L,s,t,R]
+ * Z[L, t, r, s*chiB + B].  W [DL, 2, 2, DR], E [DR, chir, chiB]. */
+int aq_mpo_env_combine(int dtype, const void* Z, const void* W, int64_t DL, int64_t DR,
+                       int64_t chir, int64_t chiB, void* E, hipStream_t stream);
+
+/* Tile-boundary renormalisation of build_left_envs / build_right_envs
+ * (src/atlas_q/sim_env_scan.py:53-130): X_b <- X_b / max(||X_b||_F, floor)
+ * for `batch` blocks of `count` contiguous elements at X + b*stride. */
+int aq_fro_normalize_batched(int dtype, void* X, int64_t batch, int64_t stride, int64_t count,
+                             double floor_, hipStream_t stream);
 
 #ifdef __cplusplus
 }
