@@ -41,7 +41,7 @@ SAMPLE_DESC = _np.dtype([("L", "u8"), ("off", "u8"), ("site", "u8"), ("Tout", "u
 COPY_DESC = _np.dtype([("src", "u8"), ("dst", "u8"), ("row_scale", "u8"), ("col_scale", "u8"),
                        ("rows", "i8"), ("cols", "i8"), ("s0", "i8"), ("s1", "i8"),
                        ("conj", "i4"), ("pad", "i4")])
-AQ_GROUPED_GEMM, AQ_GROUPED_THETA = 0, 1
+AQ_GROUPED_GEMM, AQ_GROUPED_THETA, AQ_GROUPED_HERK = 0, 1, 2
 
 
 class RankResult(ctypes.Structure):

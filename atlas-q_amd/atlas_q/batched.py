@@ -210,8 +210,8 @@ class BatchedMPS:
         gram["conj_b"] = np.where(left, 1, 0)
         gram["c_s0"], gram["c_s1"] = gld, 1
         gram["kind"] = _ffi.AQ_GROUPED_GEMM
-        self._grouped_gemm(_ffi.AQ_C128, gram[left], 1, "gram")
-        self._grouped_gemm(_ffi.AQ_C128, gram[~left], 2, "gram")
+        self._grouped_gemm(_ffi.AQ_C128, gram[left], 1 | 4, "gram")
+        self._grouped_gemm(_ffi.AQ_C128, gram[~left], 2 | 4, "gram")
         # ---- eigen-decompositions per bucket (descending)
         W_all, V_all = {}, {}
         for sz, (idx, T) in buckets.items():
@@ -271,8 +271,8 @@ class BatchedMPS:
         pj["b_s1"] = 1
         pj["c_s0"] = np.where(left, n, k)
         pj["c_s1"] = 1
-        self._grouped_gemm(_ffi.AQ_C128, pj[left], 2, "proj")
-        self._grouped_gemm(_ffi.AQ_C128, pj[~left], 3, "proj")
+        self._grouped_gemm(_ffi.AQ_C128, pj[left], 2 | 4, "proj")
+        self._grouped_gemm(_ffi.AQ_C128, pj[~left], 3 | 4, "proj")
         # new shapes: site b -> [dl, 2, k], site b+1 -> [k, 2, dr]
         cp = np.zeros(2 * G, COPY_DESC)
         A_, B_ = cp[:G], cp[G:]

@@ -56,6 +56,7 @@ class FactorStats:
 
 
 RSVD_ATTEMPTS = 2      # a rejected sketch is retried once at twice the width
+GROUPED_GRAM = True    # factor a sub-layer's Gram cores with grouped launches
 RETRY_REASONS = ("tail", "uncertified", "rank_at_width", "orth")
 
 _DEFAULT = {}
@@ -198,6 +199,116 @@ def _harvest(mps, feats, k):
         mps.harvest.append((feats.cpu().numpy(), k))
 
 
+def _rank_batched(S32: torch.Tensor, caps, eps: float):
+    """Rank rule (aq_rank_select semantics) on every row of S32 [B, m] in one
+    launch; one device->host copy of the B results."""
+    from ._ffi import RankResult, check, lib
+    import ctypes
+    B, m = S32.shape
+    dev = S32.device
+    ptrs = torch.tensor([S32[b].data_ptr() for b in range(B)], dtype=torch.int64).to(dev)
+    mm = torch.full((B,), m, dtype=torch.int64).to(dev)
+    cp = torch.tensor([int(c) for c in caps], dtype=torch.int64).to(dev)
+    sz = ctypes.sizeof(RankResult)
+    out = torch.empty(B * sz, dtype=torch.uint8, device=dev)
+    check(lib().aq_rank_select_batched(0, K._ptr(ptrs), K._ptr(mm), K._ptr(cp), float(eps), B,
+                                       K._ptr(out), K._stream(S32)), "aq_rank_select_batched")
+    host = out.cpu().numpy().tobytes()
+    return [RankResult.from_buffer_copy(host[b * sz:(b + 1) * sz]) for b in range(B)]
+
+
+def _gram_group(mps, jobs, st):
+    """Exact Gram factorization of a group of complex64 updates with m <= n
+    and one order m, every step one launch for the whole group: the FP64
+    Gram matrices (lower-triangle tiles only: the eigensolver reads that
+    triangle) in one grouped GEMM, one batched rocSOLVER eigensolve, the rank
+    rule for all, the projections Vh_k = U_k^H X in one grouped GEMM that
+    reads the eigensolver's output rows directly, and the cores (A' = U_k
+    S_k, B' = S_k^-1 Vh_k, rounded to complex64) in one grouped copy.
+    Returns per-job results, or None for a job that needs the per-job path
+    (eigensolver failure, precision promotion)."""
+    import ctypes
+    from ._ffi import AQ_GROUPED_HERK, COPY_DESC, check, lib
+    from .eig import heevd_batched
+    c128 = torch.complex128
+    B = len(jobs)
+    m = jobs[0].X.shape[0]
+    dev = jobs[0].X.device
+    keep = []
+    X64 = [j.X.to(c128) for j in jobs]
+    Gb = torch.zeros((B, m, m), dtype=c128, device=dev)
+    gd = np.stack([K.gemm_desc(X64[b], X64[b].T, Gb[b], conj_b=True, kind=AQ_GROUPED_HERK)
+                   for b in range(B)])
+    work = float(sum(4.0 * m * m * x.shape[1] for x in X64))
+    with RL.timed("gram_fp64_herk_grouped", work, "fp64_mfma"):
+        K.grouped_gemm_descs(gd, c128, dev, keep)
+    with RL.timed("eig_zheevd_batched", B * 16.0 * m ** 3 / 3.0, "hbm"):
+        lam, _, ok = heevd_batched(Gb, inplace=True, return_ok=True)
+    S64 = torch.sqrt(torch.clamp(lam, min=0.0)).contiguous()          # [B, m] descending
+    ranks = _rank_batched(S64.to(torch.float32).contiguous(), [j.cap for j in jobs],
+                          mps.eps_bond)
+    out = [None] * B
+    good = []
+    for b, (j, r) in enumerate(zip(jobs, ranks)):
+        if not ok[b]:
+            continue
+        if r.flags & 1:
+            k, e, sv, c = 1, 0.0, 0.0, float("inf")
+        else:
+            k, e, sv, c = int(r.k), float(r.eps_local), float(r.entropy), float(r.condS)
+        if math.isfinite(c) and c > mps.dtype_policy.promote_if_cond_gt:
+            continue                                   # per-job path promotes
+        good.append((b, k, e, sv, c))
+    if not good:
+        return out
+    inv = torch.where(S64 > 0, 1.0 / S64, torch.zeros_like(S64))
+    ks = np.array([g[1] for g in good], np.int64)
+    ns = np.array([X64[g[0]].shape[1] for g in good], np.int64)
+    off = np.zeros(len(good), np.int64)
+    off[1:] = np.cumsum((ks * ns)[:-1])
+    P = torch.empty(int((ks * ns).sum()), dtype=c128, device=dev)
+    pd = []
+    for (b, k, _, _, _), o, n in zip(good, off, ns):
+        Vh = P[o:o + k * n].view(k, n)
+        # U_k^H row i = conj(v_i) = row m-1-i of the eigensolver output
+        pd.append(K.gemm_desc(Gb[b], X64[b], Vh, M=k, K=m, N=n,
+                              a_ptr=Gb[b, m - 1].data_ptr(), a_strides=(-m, 1)))
+    work = float(sum(8.0 * k * m * n for k, n in zip(ks, ns)))
+    with RL.timed("proj_fp64_grouped", work, "fp64_mfma"):
+        K.grouped_gemm_descs(np.stack(pd), c128, dev, keep)
+    cps = np.zeros(2 * len(good), COPY_DESC)
+    res = []
+    for q, ((b, k, e, sv, c), o, n) in enumerate(zip(good, off, ns)):
+        j = jobs[b]
+        A_new = torch.empty((j.cl, 2, k), dtype=torch.complex64, device=dev)
+        B_new = torch.empty((k, 2, j.cr), dtype=torch.complex64, device=dev)
+        a, bb = cps[2 * q], cps[2 * q + 1]
+        # A'[r, c] = U[r, c] S[c], U[r, c] = conj(output[m-1-c, r])
+        a["src"], a["dst"] = Gb[b, m - 1].data_ptr(), A_new.data_ptr()
+        a["col_scale"] = S64[b].data_ptr()
+        a["rows"], a["cols"], a["s0"], a["s1"], a["conj"] = m, k, 1, -m, 1
+        # B'[r, c] = Vh[r, c] / S[r]
+        bb["src"], bb["dst"] = P[o:].data_ptr(), B_new.data_ptr()
+        bb["row_scale"] = inv[b].data_ptr()
+        bb["rows"], bb["cols"], bb["s0"], bb["s1"] = k, n, n, 1
+        res.append((b, (A_new, B_new, k, e, sv, c, "gram_fp64", j.local)))
+    cnt = cps["rows"] * cps["cols"]
+    pre = np.zeros(len(cnt), np.int64)
+    pre[1:] = np.cumsum(cnt[:-1])
+    dd = torch.from_numpy(cps.view(np.uint8).copy()).to(dev)
+    pp = torch.from_numpy(pre).to(dev)
+    total = int(cnt.sum())
+    check(lib().aq_grouped_copy(K._ptr(dd), K._ptr(pp), len(cps), total,
+                                int(min(-(-total // 256), 65536)), K._stream(Gb)),
+          "aq_grouped_copy")
+    keep += [dd, pp, P, inv, S64, Gb, X64]
+    # the temporaries are released in stream order (one stream), so the
+    # caching allocator cannot hand them out before the launches above ran
+    for b, r in res:
+        out[b] = r
+    return out
+
+
 def finish_gram(mps, jobs):
     """Exact Gram factorization of several updates.  Gram matrices of equal
     order are factored together: orders <= 64 in the batched Jacobi kernel,
@@ -212,7 +323,19 @@ def finish_gram(mps, jobs):
         orders.setdefault(min(j.X.shape), []).append(idx)
     grams = [None] * len(jobs)       # (G or None, left, X64)
     eig = [None] * len(jobs)
+    done = {}
     for n, idxs in orders.items():
+        grouped = [i for i in idxs if jobs[i].X.shape[0] <= jobs[i].X.shape[1]
+                   and jobs[i].X.dtype == torch.complex64]
+        if (n > JACOBI_MAX and len(grouped) > 1 and mps.budget_global_mb is None
+                and GROUPED_GRAM):
+            res = _gram_group(mps, [jobs[i] for i in grouped], st)
+            for i, r in zip(grouped, res):
+                if r is not None:
+                    done[i] = r
+            idxs = [i for i in idxs if i not in done]
+            if not idxs:
+                continue
         if n <= JACOBI_MAX or len(idxs) == 1:
             for idx in idxs:
                 grams[idx] = _gram_matrix(jobs[idx].X)
@@ -234,6 +357,11 @@ def finish_gram(mps, jobs):
                 eig[idx] = (lam[b], V[b])
     outs = []
     for idx, j in enumerate(jobs):
+        if idx in done:
+            st.bump("gram")
+            outs.append(done[idx])
+            _harvest(mps, j.feats, done[idx][2])
+            continue
         G, left, X64 = grams[idx]
         if eig[idx] is None:
             try:

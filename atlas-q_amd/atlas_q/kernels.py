@@ -342,40 +342,75 @@ def fro_normalize_(X: torch.Tensor, floor: float = 1e-12) -> torch.Tensor:
     return X
 
 
+def herk_tiles(M):
+    """Tiles of an AQ_GROUPED_HERK problem of order M (numpy-vectorised)."""
+    tm = -(-np.asarray(M, np.int64) // 64)
+    q = tm // 2
+    return q * (q + 1) + (tm % 2) * (q + 1)
+
+
+def gemm_desc(A: torch.Tensor, B: torch.Tensor, C: torch.Tensor, conj_a=False, conj_b=False,
+              kind: int = _ffi.AQ_GROUPED_GEMM, M=None, N=None, K=None, a_ptr=None,
+              a_strides=None):
+    """One aq_gemm_desc_t (numpy record) for C = op(A) op(B); A may be given
+    by pointer and strides (e.g. a reversed row order) with explicit sizes."""
+    d = np.zeros(1, _ffi.GEMM_DESC)[0]
+    d["A"] = A.data_ptr() if a_ptr is None else a_ptr
+    d["B"], d["C"] = B.data_ptr(), C.data_ptr()
+    d["M"] = A.shape[0] if M is None else M
+    d["K"] = A.shape[1] if K is None else K
+    d["N"] = B.shape[1] if N is None else N
+    d["a_s0"], d["a_s1"] = A.stride() if a_strides is None else a_strides
+    d["b_s0"], d["b_s1"] = B.stride()
+    d["c_s0"], d["c_s1"] = C.stride()
+    d["conj_a"], d["conj_b"], d["kind"] = int(conj_a), int(conj_b), int(kind)
+    return d
+
+
+def grouped_gemm_descs(descs: np.ndarray, dtype: torch.dtype, device, keep=None,
+                       xcd_ranges: bool = False) -> None:
+    """Launch prepared descriptors (one dtype, one operand layout) in one
+    grouped launch; the uploads are appended to ``keep`` (the caller holds
+    them until the stream has run) or synchronised here."""
+    if len(descs) == 0:
+        return
+    ka = bool(np.all(descs["a_s1"] == 1))
+    nb = bool(np.all(descs["b_s1"] == 1))
+    if not ka and np.any(descs["a_s1"] == 1) or not nb and np.any(descs["b_s1"] == 1):
+        raise ValueError("grouped_gemm: operands must share their layouts")
+    herk = descs["kind"] == _ffi.AQ_GROUPED_HERK
+    tiles = np.where(herk, herk_tiles(descs["M"]),
+                     (-(-descs["M"] // 64)) * (-(-descs["N"] // 128)))
+    pre = np.zeros(len(tiles), np.int64)
+    pre[1:] = np.cumsum(tiles[:-1])
+    total = int(tiles.sum())
+    dd = torch.from_numpy(np.ascontiguousarray(descs).view(np.uint8).copy()).to(device)
+    pd = torch.from_numpy(pre).to(device)
+    grid = int(min(total, 65536))
+    if xcd_ranges and grid >= 8:
+        grid -= grid % 8          # XCD-contiguous tile ranges need a multiple of 8
+    st = torch.cuda.current_stream(device)
+    no_theta = not bool(np.any(descs["kind"] == _ffi.AQ_GROUPED_THETA))
+    check(lib().aq_grouped_gemm(_code(dtype), ctypes.c_void_p(dd.data_ptr()),
+                                ctypes.c_void_p(pd.data_ptr()), len(descs), total,
+                                int(ka) | (int(nb) << 1) | (int(no_theta) << 2)
+                                | (int(bool(xcd_ranges)) << 3), grid,
+                                ctypes.c_void_p(st.cuda_stream)), "aq_grouped_gemm")
+    if keep is None:
+        st.synchronize()
+    else:
+        keep.extend([dd, pd])
+
+
 def grouped_gemm(problems, dtype: torch.dtype, device) -> None:
     """C_p = op(A_p) op(B_p) for a list of (A, B, C, conj_a, conj_b) 2-D
     tensor views in one grouped launch (all of one dtype; every A shares its
     fast axis, as every B does)."""
     if not problems:
         return
-    descs = np.zeros(len(problems), _ffi.GEMM_DESC)
-    ka = nb = None
-    for i, (A, B, C, ca, cb) in enumerate(problems):
+    descs = np.stack([gemm_desc(A, B, C, ca, cb) for A, B, C, ca, cb in problems])
+    for A, B, C, _, _ in problems:
         assert not A.is_conj() and not B.is_conj() and C.dtype == dtype
-        d = descs[i]
-        d["A"], d["B"], d["C"] = A.data_ptr(), B.data_ptr(), C.data_ptr()
-        d["M"], d["K"] = A.shape
-        d["N"] = B.shape[1]
-        d["a_s0"], d["a_s1"] = A.stride()
-        d["b_s0"], d["b_s1"] = B.stride()
-        d["c_s0"], d["c_s1"] = C.stride()
-        d["conj_a"], d["conj_b"], d["kind"] = int(ca), int(cb), _ffi.AQ_GROUPED_GEMM
-        a_fast, b_fast = A.stride(1) == 1, B.stride(1) == 1
-        if ka is None:
-            ka, nb = a_fast, b_fast
-        elif (ka, nb) != (a_fast, b_fast):
-            raise ValueError("grouped_gemm: operands must share their layouts")
-    tiles = (-(-descs["M"] // 64)) * (-(-descs["N"] // 128))
-    pre = np.zeros(len(tiles), np.int64)
-    pre[1:] = np.cumsum(tiles[:-1])
-    total = int(tiles.sum())
-    dd = torch.from_numpy(descs.view(np.uint8).copy()).to(device)
-    pd = torch.from_numpy(pre).to(device)
-    grid = int(min(total, 65536))
-    check(lib().aq_grouped_gemm(_code(dtype), ctypes.c_void_p(dd.data_ptr()),
-                                ctypes.c_void_p(pd.data_ptr()), len(descs), total,
-                                int(ka) | (int(nb) << 1), grid,
-                                ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)),
-          "aq_grouped_gemm")
-    # keep the descriptor uploads alive until the launch has consumed them
-    torch.cuda.current_stream(device).synchronize()
+    grouped_gemm_descs(descs, dtype, device)
+
+

@@ -12,6 +12,8 @@
 
 namespace aq {
 
+constexpr int kXcds = 8;  // accelerator complex dies of an MI355X
+
 __device__ __forceinline__ int find_owner(const int64_t* __restrict__ prefix, int P, int64_t t) {
   int lo = 0, hi = P - 1;
   while (lo < hi) {  // largest p with prefix[p] <= t
@@ -23,20 +25,49 @@ __device__ __forceinline__ int find_owner(const int64_t* __restrict__ prefix, in
 }
 
 // ------------------------------------------------------------ grouped GEMM
-template <typename T, bool KFastA, bool NFastB>
-__global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(const aq_gemm_desc_t* __restrict__ d,
+// WithTheta = false: the group holds no AQ_GROUPED_THETA problem and the gate
+// epilogue is not compiled in (it doubles the kernel's register footprint
+// and halves its occupancy).
+template <typename T, bool KFastA, bool NFastB, bool WithTheta>
+__global__ __launch_bounds__(kThreads, 2) void grouped_gemm_kernel(const aq_gemm_desc_t* __restrict__ d,
                                                                 const int64_t* __restrict__ prefix,
-                                                                int P, int64_t total) {
+                                                                int P, int64_t total,
+                                                                int xcd_ranges) {
   __shared__ Smem<T> sm;
-  for (int64_t t = blockIdx.x; t < total; t += gridDim.x) {
+  // Work-groups are dispatched to the 8 XCDs round-robin (XCD = blockIdx.x
+  // mod 8) and every XCD has its own L2.  With xcd_ranges (and a grid that is
+  // a multiple of 8) each XCD takes a contiguous range of tiles, so the tiles
+  // that run together on one XCD are neighbours of one problem and share
+  // operand strips in its L2; otherwise the tiles are strided over the grid.
+  int64_t t0 = blockIdx.x, t_end = total, t_step = gridDim.x;
+  if (xcd_ranges && gridDim.x % kXcds == 0) {
+    const int64_t chunk = cdiv(total, kXcds);
+    t0 = (int64_t)(blockIdx.x % kXcds) * chunk + blockIdx.x / kXcds;
+    t_end = min(total, (int64_t)(blockIdx.x % kXcds + 1) * chunk);
+    t_step = gridDim.x / kXcds;
+  }
+  for (int64_t t = t0; t < t_end; t += t_step) {
     const int p = find_owner(prefix, P, t);
     const aq_gemm_desc_t g = d[p];
     const int64_t local = t - prefix[p];
-    const int paired = g.kind == AQ_GROUPED_THETA;
+    const int paired = WithTheta && g.kind == AQ_GROUPED_THETA;
     const int64_t half = paired ? g.N / 2 : g.N;
     const int64_t tn = paired ? cdiv(half, kBJ) : cdiv(g.N, kBN);
-    const int64_t m0 = (local / tn) * kBM;
-    const int64_t j0 = (local % tn) * kBJ;
+    int64_t m0, j0;
+    if (g.kind == AQ_GROUPED_HERK) {
+      // tiles meeting the lower triangle of a square C: tile row bm holds
+      // floor(bm/2) + 1 tiles, so rows (2q, 2q+1) hold 2(q+1) tiles and the
+      // tiles before them number q(q+1)
+      int64_t q = (int64_t)((sqrt(4.0 * (double)local + 1.0) - 1.0) * 0.5);
+      while (q > 0 && q * (q + 1) > local) --q;
+      while ((q + 1) * (q + 2) <= local) ++q;
+      const int64_t r = local - q * (q + 1);
+      m0 = (2 * q + r / (q + 1)) * kBM;
+      j0 = (r % (q + 1)) * kBJ;
+    } else {
+      m0 = (local / tn) * kBM;
+      j0 = (local % tn) * kBJ;
+    }
     AStrided<T> aop{(const c2_t<T>*)g.A, g.a_s0, g.a_s1, g.M, g.K, g.conj_a};
     BStrided<T> bop{(const c2_t<T>*)g.B, g.b_s0, g.b_s1, g.K, g.N, g.conj_b};
     ColMap cm{paired, half};
@@ -44,6 +75,7 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(const aq_gemm_de
     __syncthreads();  // LDS reuse across tiles
     cgemm_mainloop<T, KFastA, NFastB>(aop, bop, cm, g.K, m0, j0, sm, fr);
     c2_t<T>* C = (c2_t<T>*)g.C;
+    if constexpr (WithTheta) {
     if (paired) {
       // theta epilogue: X[2l+p, q*chiR + j] = sum U[2p+q, 2p1+p2] Psi[2l+p1, p2*chiR + j]
       using M = Mfma<T>;
@@ -90,18 +122,19 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(const aq_gemm_de
                 C[(row + pp) * g.c_s0 + (q * half + j) * g.c_s1] = v;
               }
           }
-    } else {
-      frag_foreach<T>(fr, [&](int row, int h, int jj, T re, T im) {
-        const int64_t m = m0 + row;
-        const int64_t n = 2 * j0 + h * kBJ + jj;
-        if (m < g.M && n < g.N) {
-          c2_t<T> v;
-          v.x = re;
-          v.y = im;
-          C[m * g.c_s0 + n * g.c_s1] = v;
-        }
-      });
+      continue;
     }
+    }
+    frag_foreach<T>(fr, [&](int row, int h, int jj, T re, T im) {
+      const int64_t m = m0 + row;
+      const int64_t n = 2 * j0 + h * kBJ + jj;
+      if (m < g.M && n < g.N) {
+        c2_t<T> v;
+        v.x = re;
+        v.y = im;
+        C[m * g.c_s0 + n * g.c_s1] = v;
+      }
+    });
   }
 }
 
@@ -129,7 +162,7 @@ __global__ void grouped_1q_kernel(const aq_site1q_desc_t* __restrict__ d,
 
 // ------------------------------------------------------------ grouped sampler
 // one site index for many MPS: tiles of every MPS's [S, chiL] x [chiL, 2 chiR]
-__global__ __launch_bounds__(kThreads) void grouped_sample_gemm_kernel(
+__global__ __launch_bounds__(kThreads, 2) void grouped_sample_gemm_kernel(
     const aq_sample_desc_t* __restrict__ d, const int64_t* __restrict__ prefix, int P,
     int64_t total) {
   __shared__ Smem<float> sm;
@@ -205,11 +238,20 @@ extern "C" int aq_grouped_gemm(int dtype, const aq_gemm_desc_t* descs, const int
   if (rc) return rc;
   AQ_REQUIRE(total_tiles > 0, AQ_ERR_ARG, "aq_grouped_gemm: no tiles");
   AQ_REQUIRE(grid >= 1 && grid <= 65536, AQ_ERR_ARG, "aq_grouped_gemm: grid out of range");
-  AQ_REQUIRE(layout >= 0 && layout <= 3, AQ_ERR_ARG, "aq_grouped_gemm: layout 0..3");
-  const bool ka = layout & 1, nb = layout & 2;
-#define AQ_GG(T, KA, NB)                                                                     \
-  grouped_gemm_kernel<T, KA, NB><<<grid, kThreads, 0, stream>>>(descs, prefix, num_problems, \
-                                                               total_tiles)
+  AQ_REQUIRE(layout >= 0 && layout <= 15, AQ_ERR_ARG, "aq_grouped_gemm: layout 0..15");
+  const bool ka = layout & 1, nb = layout & 2, no_theta = layout & 4;
+  const int xcd = (layout & 8) ? 1 : 0;
+#define AQ_GG(T, KA, NB)                                                                       \
+  do {                                                                                         \
+    if (no_theta)                                                                              \
+      grouped_gemm_kernel<T, KA, NB, false><<<grid, kThreads, 0, stream>>>(descs, prefix,      \
+                                                                           num_problems,       \
+                                                                           total_tiles, xcd);  \
+    else                                                                                       \
+      grouped_gemm_kernel<T, KA, NB, true><<<grid, kThreads, 0, stream>>>(descs, prefix,       \
+                                                                          num_problems,        \
+                                                                          total_tiles, xcd);   \
+  } while (0)
   if (dtype == AQ_C64) {
     if (ka && nb) AQ_GG(float, true, true);
     else if (ka) AQ_GG(float, true, false);

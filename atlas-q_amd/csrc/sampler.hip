@@ -20,7 +20,7 @@
 namespace aq {
 
 template <typename T>
-__global__ __launch_bounds__(kThreads) void sample_gemm_kernel(
+__global__ __launch_bounds__(kThreads, 2) void sample_gemm_kernel(
     const c2_t<T>* __restrict__ Lbase, const int32_t* __restrict__ off, int64_t ldL,
     const c2_t<T>* __restrict__ site, c2_t<T>* __restrict__ Tout, T* __restrict__ part,
     int64_t S, int64_t chiL, int64_t chiR, int ntiles) {

@@ -66,7 +66,7 @@ __device__ __forceinline__ void theta_epilogue(const FR& fr, const Gate4<T>& g, 
 }
 
 template <typename T>
-__global__ __launch_bounds__(kThreads) void theta_kernel(const c2_t<T>* __restrict__ A,
+__global__ __launch_bounds__(kThreads, 2) void theta_kernel(const c2_t<T>* __restrict__ A,
                                                          const c2_t<T>* __restrict__ B,
                                                          Gate4<T> g, c2_t<T>* __restrict__ X,
                                                          int64_t chiL, int64_t chiM,
@@ -97,7 +97,7 @@ __global__ __launch_bounds__(kThreads) void theta_bf16x3_kernel(
 }
 
 template <typename T, bool KFastA, bool NFastB>
-__global__ __launch_bounds__(kThreads) void cgemm_kernel(AStrided<T> aop, BStrided<T> bop,
+__global__ __launch_bounds__(kThreads, 2) void cgemm_kernel(AStrided<T> aop, BStrided<T> bop,
                                                          c2_t<T>* __restrict__ C, int64_t M,
                                                          int64_t N, int64_t K, int64_t c_s0,
                                                          int64_t c_s1) {
@@ -142,7 +142,7 @@ __global__ __launch_bounds__(kThreads) void cgemm_bf16x3_kernel(AStrided<float> 
 // reduction slice [z*chunk, (z+1)*chunk) into part[z][M][N]; the reduce kernel
 // adds the slices in order (deterministic) into C.
 template <typename T, bool KFastA, bool NFastB, bool kBf16x3>
-__global__ __launch_bounds__(kThreads) void cgemm_splitk_kernel(AStrided<T> aop, BStrided<T> bop,
+__global__ __launch_bounds__(kThreads, kBf16x3 ? 1 : 2) void cgemm_splitk_kernel(AStrided<T> aop, BStrided<T> bop,
                                                                 c2_t<T>* __restrict__ part,
                                                                 int64_t M, int64_t N, int64_t K,
                                                                 int64_t chunk) {

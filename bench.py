@@ -21,8 +21,9 @@ plain        : updates/s of the reference-compatible plain path (FP32-MFMA
                theta + torch.linalg.svd) on the same slice and gates
 fidelity     : per-update truncation fidelity of both paths against an FP64
                truncation of the same input (complex128 SVD), |dF| reported
-roofline     : the FP64 Gram GEMM (the hand-written kernel that dominates the
-               path's own time), achieved FP64 TFLOP/s from HIP events
+roofline     : the dominant single launch of the library's own (a grouped FP64
+               MFMA launch over the sub-layer: the Gram matrices or the
+               projections), achieved FP64 TFLOP/s from HIP events
 cpu_baseline : the CPU oracle (torch-CPU restatement of the reference update)
                on the host cores, a bounded sample of updates at the same shape
 sampler      : shots/s of the fused sweep sampler on the plain-GPU circuit's
@@ -292,15 +293,16 @@ def marker(torch):
     torch.cuda.synchronize()
 
 
-def traffic_from_profile():
-    """HBM bytes per launch of the FP64 Gram GEMM at this shape, from the
-    committed counter summary (FETCH_SIZE and WRITE_SIZE runs, calibrated on a
-    copy of known size); None if the summary is absent."""
-    path = ROOT / "profiles" / "gram_fp64_cgemm_chi1024_traffic.md"
+def traffic_from_profile(kernel: str):
+    """HBM bytes per launch of the update path's grouped FP64 launch `kernel`
+    at the bench's shape, from the committed counter summary (FETCH_SIZE and
+    WRITE_SIZE runs of ``bench.py --gemm-probe``, calibrated on a copy of
+    known size); None if the summary has no figure for it."""
+    path = ROOT / "profiles" / "grouped_fp64_gemm_chi1024_traffic.md"
     if not path.exists():
         return None, None
     for line in path.read_text().splitlines():
-        if line.startswith("traffic_bytes_per_launch"):
+        if line.startswith(f"traffic_bytes_per_launch[{kernel}]"):
             return float(line.split("=")[1]), str(path.relative_to(ROOT))
     return None, None
 
@@ -316,14 +318,27 @@ def main():
     import torch
     from atlas_q import kernels as K
     if a.gemm_probe:
-        from atlas_q.adaptive_mps import _gate4
-        ts = slice_state(torch, 2, a.chi, a.seed, "cuda")
-        X = K.theta(ts[0], ts[1], _gate4(gate_schedule(2, 1, a.seed + 1)[0][1]))
-        X64 = X.to(torch.complex128)
-        for _ in range(5):
-            G = K.cgemm(X64, X64.T, conj_b=True)
+        # the update path's grouped FP64 launches at the bench's shape: the
+        # lower-triangle Gram X X^H of 64 thetas 2048 x 2048, then the 64
+        # projections U_k^H X (k = 1024) reading eigenvector rows in place
+        import numpy as np
+        from atlas_q._ffi import AQ_GROUPED_HERK
+        B, m, k = 64, 2 * a.chi, a.chi
+        gen = torch.Generator(device="cuda").manual_seed(a.seed)
+        X = torch.randn((B, m, m), dtype=torch.complex128, device="cuda", generator=gen)
+        G = torch.zeros((B, m, m), dtype=torch.complex128, device="cuda")
+        V = torch.empty((B, k, m), dtype=torch.complex128, device="cuda")
+        gd = np.stack([K.gemm_desc(X[b], X[b].T, G[b], conj_b=True, kind=AQ_GROUPED_HERK)
+                       for b in range(B)])
+        pd = np.stack([K.gemm_desc(G[b], X[b], V[b], M=k, K=m, N=m,
+                                   a_ptr=G[b, m - 1].data_ptr(), a_strides=(-m, 1))
+                       for b in range(B)])
+        for _ in range(3):
+            K.grouped_gemm_descs(gd, torch.complex128, "cuda")
+        for _ in range(3):
+            K.grouped_gemm_descs(pd, torch.complex128, "cuda")
         torch.cuda.synchronize()
-        print(json.dumps({"gemm_probe": list(G.shape), "launches": 5}), flush=True)
+        print(json.dumps({"gemm_probe": [B, m, k], "launches": [3, 3]}), flush=True)
         return
     if a.copy_probe:
         src = torch.ones(a.copy_probe << 18, dtype=torch.float32, device="cuda")
@@ -362,8 +377,8 @@ def main():
     ups = a.steps / t_fast
     log(f"fast path: {ups:.2f} updates/s")
     roof = RL.summary()
-    if roof and roof["kernel"] == "gram_fp64_cgemm" and chi == 1024:
-        roof["traffic"], roof["traffic_source"] = traffic_from_profile()
+    if roof and chi == 1024 and a.sites == 128:
+        roof["traffic"], roof["traffic_source"] = traffic_from_profile(roof["kernel"])
     eig = None
     if roof and "eig_zheevd_batched" in roof["all"]:
         e = roof["all"]["eig_zheevd_batched"]

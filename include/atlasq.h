@@ -199,6 +199,10 @@ int aq_mlp_adamw(float* master, const float* grad, float* m, float* v, uint16_t*
 #define AQ_GROUPED_GEMM 0  /* C = op(A) op(B), linear columns */
 #define AQ_GROUPED_THETA 1 /* theta: A [chiL,2,chiM], B [chiM,2,chiR], C = X,
                               M = 2chiL, N = 2chiR, K = chiM, gate in `gate` */
+#define AQ_GROUPED_HERK 2  /* C = op(A) op(B) with M == N, only the tiles that meet
+                              the lower triangle (row >= col) are computed: the
+                              Gram matrix X X^H where the eigensolver reads one
+                              triangle (aq_heevd_batched reads row >= col) */
 typedef struct {
   const void* A;
   const void* B;
@@ -208,9 +212,16 @@ typedef struct {
   int32_t conj_a, conj_b, kind, pad;
   double gate[32]; /* AQ_GROUPED_THETA: U[4][4] interleaved re/im, row-major */
 } aq_gemm_desc_t;
-/* Tiles of a problem: ceil(M/64) * ceil(N/128) (GEMM) or ceil(M/64) *
- * ceil(N/2/64) (theta).  layout: bit0 = A unit stride along k (a_s1 == 1),
- * bit1 = B unit stride along n (b_s1 == 1), uniform over the group. */
+/* Tiles of a problem: ceil(M/64) * ceil(N/128) (GEMM), ceil(M/64) *
+ * ceil(N/2/64) (theta) or sum_{bm < ceil(M/64)} (floor(bm/2) + 1) (HERK).
+ * layout: bit0 = A unit stride along k (a_s1 == 1), bit1 = B unit stride
+ * along n (b_s1 == 1), uniform over the group; bit2 = the group holds no
+ * AQ_GROUPED_THETA problem (selects a kernel without the gate epilogue, at
+ * twice the occupancy; a THETA problem in such a group is computed as a
+ * plain GEMM); bit3 = XCD-contiguous tile ranges: with a grid that is a
+ * multiple of 8 each of the 8 XCDs (round-robin dispatch) takes a contiguous
+ * range of tiles (fewer L2 misses; measured slightly slower for the FP64
+ * Gram / projection groups, so off by default). */
 int aq_grouped_gemm(int dtype, const aq_gemm_desc_t* descs, const int64_t* prefix,
                     int num_problems, int64_t total_tiles, int layout, int grid,
                     hipStream_t stream);

@@ -183,3 +183,36 @@ def test_apply_1q_transposed_convention(dtype):
     ref = torch.einsum("st,asb->atb", U, T)
     out = K.apply_1q(T.to(DEV), K.gate_to_host(U, (2, 2)))
     assert _relerr(out, ref) < TOL[dtype]
+
+
+@pytest.mark.parametrize("xcd", [False, True])
+@pytest.mark.parametrize("M,N", [(128, 256), (100, 73), (200, 200), (130, 520)])
+def test_grouped_herk_lower_triangle(M, N, xcd):
+    """AQ_GROUPED_HERK computes every element with row >= col of X X^H (the
+    triangle the eigensolver reads); problems of different orders in one
+    launch."""
+    from atlas_q._ffi import AQ_GROUPED_HERK
+    torch.manual_seed(M + N)
+    Xs = [torch.randn(M, N, dtype=torch.complex128, device=DEV),
+          torch.randn(M // 2 + 1, N, dtype=torch.complex128, device=DEV)]
+    Gs = [torch.zeros(x.shape[0], x.shape[0], dtype=torch.complex128, device=DEV) for x in Xs]
+    descs = np.stack([K.gemm_desc(x, x.T, g, conj_b=True, kind=AQ_GROUPED_HERK)
+                      for x, g in zip(Xs, Gs)])
+    K.grouped_gemm_descs(descs, torch.complex128, DEV, xcd_ranges=xcd)
+    for x, g in zip(Xs, Gs):
+        ref = x @ x.conj().T
+        low = torch.tril(torch.ones_like(ref.real, dtype=torch.bool))
+        assert torch.allclose(g[low], ref[low], atol=1e-10 * N)
+
+
+def test_heevd_reads_only_the_lower_triangle():
+    """The row-major lower triangle is all aq_heevd_batched reads (column-major
+    upper): the other triangle may hold anything."""
+    from atlas_q.eig import heevd_batched
+    rng = np.random.default_rng(3)
+    Z = rng.standard_normal((2, 96, 96)) + 1j * rng.standard_normal((2, 96, 96))
+    G = Z @ np.conj(np.transpose(Z, (0, 2, 1)))
+    Gl = np.tril(G) + np.triu(np.full_like(G, 7.0 + 3.0j), 1)
+    lam, V = heevd_batched(torch.from_numpy(Gl).to(DEV))
+    ref = np.linalg.eigvalsh(G)[:, ::-1]
+    assert np.abs(lam.cpu().numpy() - ref).max() < 1e-10 * np.abs(ref).max()
