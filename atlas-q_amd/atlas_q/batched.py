@@ -34,7 +34,8 @@ import torch
 from . import _ffi
 from ._ffi import (COPY_DESC, GEMM_DESC, SAMPLE_DESC, SITE1Q_DESC, RankResult, check, lib)
 from .autotune import Autotuner
-from .eig import eigh_desc
+from .eig import JACOBI_MAX, eigh_desc, finite_rows, heevd_batched
+from .linalg_robust import robust_svd
 
 C64 = torch.complex64
 C128 = torch.complex128
@@ -61,6 +62,40 @@ def _prefix(counts: np.ndarray) -> Tuple[np.ndarray, int]:
 
 def _cdiv(a, b):
     return -(-a // b)
+
+
+def _condition_bucket(T: torch.Tensor, orders: np.ndarray, jitter: torch.Tensor):
+    """Prepare a size bucket for rocSOLVER's divide-and-conquer.
+
+    Matrix b holds its Gram matrix in the leading orders[b] rows/columns and
+    zeros elsewhere; d_b is its largest diagonal entry (1 when it is zero).
+    * The padding diagonal gets the distinct negative values
+      -d_b (1 + i / size): the padding eigenvalues sort below every Gram
+      eigenvalue (which is >= 0) instead of forming one large zero cluster
+      with the Gram's own null space.
+    * 1e-13 d_b ``jitter`` (a fixed Hermitian matrix of spectral norm ~1) is
+      added: on the exactly degenerate spectra of structured states
+      (CZ-ladder circuits) the strided-batched zheevd returns non-finite
+      eigenvectors without an error code; the perturbation moves every
+      eigenvalue by at most ~1e-13 d_b, as the reference's 1e-12 SVD jitter
+      retry does (linalg_robust.py:31-39)."""
+    B, sz, _ = T.shape
+    dg = T.diagonal(dim1=1, dim2=2)
+    d = dg.real.amax(1, keepdim=True)
+    d = torch.where(d > 0, d, torch.ones_like(d))
+    if int(orders.min()) < sz:
+        i = torch.arange(sz, device=T.device, dtype=torch.float64)
+        pad = torch.from_numpy(np.arange(sz)[None, :] >= orders[:, None]).to(T.device)
+        dg.copy_(torch.where(pad, (-d * (1.0 + i / sz)).to(T.dtype), dg))
+    if jitter is not None:
+        T.add_(jitter.unsqueeze(0) * (1e-13 * d).unsqueeze(-1).to(T.dtype))
+
+
+def _jitter_matrix(sz: int, device) -> torch.Tensor:
+    """Fixed Hermitian matrix of order sz with spectral norm ~1 (seeded by sz)."""
+    g = torch.Generator(device=device).manual_seed(1000003 + sz)
+    R = torch.randn((sz, sz), dtype=torch.complex128, device=device, generator=g)
+    return (R + R.conj().T) * (0.5 / math.sqrt(2.0 * sz))
 
 
 class BatchedMPS:
@@ -97,6 +132,8 @@ class BatchedMPS:
         self.arena[first] = 1.0
         self.tuner = autotuner if autotuner is not None else Autotuner.default()
         self.logs = {"mps": [], "bond": [], "k": [], "eps_local": [], "entropy": []}
+        self.eig_fallbacks = 0     # matrices whose batched eigensolve failed
+        self._jitter = {}          # order -> fixed Hermitian perturbation
         self._base = self.arena.data_ptr()
 
     # ------------------------------------------------------------ views
@@ -215,8 +252,20 @@ class BatchedMPS:
         # ---- eigen-decompositions per bucket (descending)
         W_all, V_all = {}, {}
         for sz, (idx, T) in buckets.items():
-            W, V = eigh_desc(T)
-            W_all[sz], V_all[sz] = W.contiguous(), V.contiguous()
+            if sz > JACOBI_MAX:
+                if sz not in self._jitter:
+                    self._jitter[sz] = _jitter_matrix(sz, self.dev)
+                _condition_bucket(T, s[idx], self._jitter[sz])
+                W, V, ok = heevd_batched(T, inplace=True, return_ok=True)
+            else:
+                W, V = eigh_desc(T)
+                ok = finite_rows(W, V).cpu().tolist()
+            W, V = W.contiguous(), V.contiguous()
+            bad = [b for b, o in enumerate(ok) if not o]
+            if bad:
+                self.eig_fallbacks += len(bad)
+                self._svd_fill(W, V, bad, idx, X64, xoff, m, n, left)
+            W_all[sz], V_all[sz] = W, V
         # ---- batched rank rule on the float32 singular values
         Sflat = torch.empty(int(s.sum()), dtype=torch.float32, device=self.dev)
         soff, _ = _prefix(s)
@@ -307,6 +356,23 @@ class BatchedMPS:
         self.logs["entropy"].extend(rr["entropy"].tolist())
         # keep the temporaries alive until the stream has consumed them
         self._keep = (X, X64, buckets, W_all, V_all, Sflat, Sf64, inv, P, dd, pd)
+
+    @staticmethod
+    def _svd_fill(W, V, bad, idx, X64, xoff, m, n, left):
+        """Eigenpairs of the bucket rows ``bad`` whose eigensolve failed, from
+        the SVD of each one's own theta: X = U S Vh gives (S^2, U) for X X^H
+        and (S^2, Vh^H) for X^H X; the rest of the row is zero."""
+        for b in bad:
+            g = int(idx[b])
+            mg, ng = int(m[g]), int(n[g])
+            X = X64[int(xoff[g]): int(xoff[g]) + mg * ng].view(mg, ng)
+            U, S, Vh, _ = robust_svd(X)
+            F = U if left[g] else Vh.conj().transpose(0, 1)
+            r = S.numel()
+            W[b].zero_()
+            V[b].zero_()
+            W[b, :r] = S * S
+            V[b, :F.shape[0], :r] = F
 
     # ------------------------------------------------------------ circuits
     def run_hea(self, circuits):

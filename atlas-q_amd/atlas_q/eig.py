@@ -9,11 +9,13 @@ brickwork sub-layer), through the library's strided-batched
 Both return eigenvalues in descending order with matching eigenvector columns.
 
 rocSOLVER can report non-convergence on strongly clustered spectra (many
-eigenvalues at zero, as in the Gram matrix of a rank-deficient theta).  As
-the reference's SVD cascade retries with a 1e-12 jitter of the input
-(linalg_robust.py:31-39), the eigensolver retries once on G + E, E a random
-Hermitian matrix of norm ~1e-13 ||G||_max, which separates the cluster and
-moves every eigenvalue by at most ||E||_2.
+eigenvalues at zero, as in the Gram matrix of a rank-deficient theta), and
+on such batches it has also returned non-finite eigenvectors with a zero
+info code; both count as a failure here.  As the reference's SVD cascade
+retries with a 1e-12 jitter of the input (linalg_robust.py:31-39),
+``eigh_desc`` retries once on G + E, E a random Hermitian matrix of norm
+~1e-13 ||G||_max, which separates the cluster and moves every eigenvalue by
+at most ||E||_2; ``heevd_batched`` reports the failed matrices to its caller.
 """
 from __future__ import annotations
 
@@ -46,12 +48,29 @@ def heevd_batched(G: torch.Tensor, inplace: bool = False, return_ok: bool = Fals
     info = torch.zeros(bsz, dtype=torch.int32, device=A.device)
     check(lib().aq_heevd_batched(n, K._ptr(A), n * n, K._ptr(W), n, K._ptr(E), K._ptr(info),
                                  bsz, K._stream(A)), "aq_heevd_batched")
-    ok = [v == 0 for v in info.cpu().tolist()]
+    good = (info == 0) & finite_rows(W, A)
+    ok = good.cpu().tolist()
     if not return_ok and not all(ok):
-        raise RuntimeError(f"aq_heevd_batched: not converged (info {info.tolist()})")
+        raise RuntimeError(f"aq_heevd_batched: not converged or non-finite (info {info.tolist()})")
     # row j of A holds conj(v_j): V = A^H, columns ascending -> flip to descending
     lam, V = W.flip(-1), A.transpose(-2, -1).conj().flip(-1)
     return (lam, V, ok) if return_ok else (lam, V)
+
+
+def finite_rows(W: torch.Tensor, V: torch.Tensor) -> torch.Tensor:
+    """Per-matrix flag [batch]: eigenvalues W [batch, n] and eigenvectors
+    V [batch, n, n] are all finite."""
+    bsz = W.shape[0]
+    return (torch.isfinite(W).all(1)
+            & torch.isfinite(torch.view_as_real(V.resolve_conj())).reshape(bsz, -1).all(1))
+
+
+def _eigh_checked(G: torch.Tensor):
+    lam, V = torch.linalg.eigh(G)
+    if not bool(torch.isfinite(lam).all()) or not bool(
+            torch.isfinite(torch.view_as_real(V)).all()):
+        raise RuntimeError("linalg.eigh: non-finite eigenpairs")
+    return lam, V
 
 
 def eigh_desc(G: torch.Tensor):
@@ -60,10 +79,10 @@ def eigh_desc(G: torch.Tensor):
         from .rank_mlp import heev_small
         return heev_small(G)
     try:
-        lam, V = torch.linalg.eigh(G)
+        lam, V = _eigh_checked(G)
     except RuntimeError:
         scale = 1e-13 * float(torch.diagonal(G, dim1=-2, dim2=-1).real.abs().max())
         E = torch.randn(G.shape, dtype=G.dtype, device=G.device)
         E = (E + E.transpose(-2, -1).conj()) * (0.5 * scale / n ** 0.5)
-        lam, V = torch.linalg.eigh(G + E)
+        lam, V = _eigh_checked(G + E)
     return lam.flip(-1), V.flip(-1)

@@ -2,7 +2,10 @@
 
 The factorizations run on the HIP device (rocSOLVER through torch).  The
 reference's last resort -- a CPU SVD/QR (linalg_robust.py:49-52, 72-76) -- is
-deliberately absent: a failure after the jitter retry raises.
+replaced by a third device attempt with a relative jitter of 1e-9 max|X|
+(rocSOLVER's SVD can fail to converge on exactly degenerate spectra, e.g.
+product-state blocks of CZ-ladder circuits, where the reference's absolute
+1e-12 jitter is not enough); a failure after it raises.
 """
 from __future__ import annotations
 
@@ -28,9 +31,16 @@ def robust_svd(X: torch.Tensor) -> Tuple[torch.Tensor, torch.Tensor, torch.Tenso
     Xj = X + (1e-12 * torch.randn_like(X))
     try:
         U, S, Vh = torch.linalg.svd(Xj, full_matrices=False)
+        return U, S, Vh, "torch_cuda_jitter"
+    except Exception:
+        pass
+    scale = float(X.abs().max()) if X.numel() else 0.0
+    Xj = X + ((1e-9 * scale) * torch.randn_like(X))
+    try:
+        U, S, Vh = torch.linalg.svd(Xj, full_matrices=False)
     except Exception as e:
-        raise RuntimeError(f"atlas_q: device SVD failed after jitter retry: {e}") from e
-    return U, S, Vh, "torch_cuda_jitter"
+        raise RuntimeError(f"atlas_q: device SVD failed after jitter retries: {e}") from e
+    return U, S, Vh, "torch_cuda_jitter_rel"
 
 
 def robust_qr(X: torch.Tensor) -> Tuple[torch.Tensor, torch.Tensor, str]:

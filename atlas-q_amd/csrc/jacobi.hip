@@ -6,7 +6,8 @@
 // pair (p, q) with a = A_pp, b = A_qq, z = A_pq = |z| e^{i phi}:
 //   tau = (b - a) / (2|z|),  t = sign(tau) / (|tau| + sqrt(1 + tau^2)),
 //   c = 1/sqrt(1+t^2), s = t c,  J = [[c, s], [-s e^{-i phi}, c e^{-i phi}]]
-// and A <- J^H A J, V <- V J.  The off-diagonal Frobenius norm is reduced
+// and A <- J^H A J, V <- V J; a pair with |z| <= 1e-18 (|a| + |b|) is not
+// rotated, only its off-diagonal cleared.  The off-diagonal Frobenius norm is reduced
 // across the work-group after every sweep; iteration stops when it falls
 // below tol * ||A||_F or after max_sweeps.  Eigenvalues are returned in
 // descending order with their eigenvectors (columns of V).
@@ -117,10 +118,19 @@ __global__ __launch_bounds__(256) void heev_jacobi_kernel(int n, const double2* 
         sm.pq[tid] = q;
         const double a = sm.a[p][p].x, bq = sm.a[q][q].x;
         const double2 z = sm.a[p][q];
-        const double az = sqrt(z.x * z.x + z.y * z.y);
+        // |z| scaled so that no square underflows: near convergence |z|
+        // reaches 1e-160 and below, where z.x^2 is subnormal, |z| loses
+        // digits and e = z/|z| is no longer unimodular (J not unitary)
+        const double zm = fmax(fabs(z.x), fabs(z.y));
+        double az = 0.0;
+        if (zm > 0.0) {
+          const double zx = z.x / zm, zy = z.y / zm;
+          az = zm * sqrt(zx * zx + zy * zy);
+        }
         double c = 1.0, s = 0.0;
         double2 e = make_double2(1.0, 0.0);
-        if (az > 1e-300 * (fabs(a) + fabs(bq) + 1e-300) && az > 0.0) {
+        // a pair already diagonal to 1e-18 (|a| + |b|) is only cleaned below
+        if (az > 1e-18 * (fabs(a) + fabs(bq)) && az > 0.0) {
           const double tau = (bq - a) / (2.0 * az);
           const double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
           c = 1.0 / sqrt(1.0 + t * t);

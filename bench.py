@@ -26,6 +26,9 @@ roofline     : the dominant single launch of the library's own (a grouped FP64
                projections), achieved FP64 TFLOP/s from HIP events
 cpu_baseline : the CPU oracle (torch-CPU restatement of the reference update)
                on the host cores, a bounded sample of updates at the same shape
+batched      : the batched form (256 uneven MPS, RY/RZ + CZ ansatz, 8 layers,
+               1000 shots each): updates/s of the grouped engine, shots/s of
+               its graph-captured sweep, and the one-at-a-time plain rate
 sampler      : shots/s of the fused sweep sampler on the plain-GPU circuit's
                shape (64 qubits, chi_max 512, 10,000 shots, complex64, a random
                right-canonical MPS at the circuit's bond profile), its FP32-MFMA
@@ -82,6 +85,14 @@ def parse():
     ap.add_argument("--sampler-reps", type=int, default=5)
     ap.add_argument("--sampler-cpu-shots", type=int, default=128)
     ap.add_argument("--no-sampler", action="store_true")
+    ap.add_argument("--batched-mps", type=int, default=256)
+    ap.add_argument("--batched-layers", type=int, default=8)
+    ap.add_argument("--batched-shots", type=int, default=1000)
+    ap.add_argument("--batched-ref", type=int, default=8,
+                    help="MPS of the population run one by one (plain method) as the "
+                         "unbatched reference rate")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batched-only", action="store_true")
     ap.add_argument("--mark", action="store_true",
                     help="launch a one-element zero-fill kernel right before and after the "
                          "timed fast-path region (dispatch markers for trace filtering)")
@@ -282,6 +293,84 @@ def sampler_bench(torch, a, dev):
     return out
 
 
+def batched_bench(torch, a, dev):
+    """The batched form: a population of independent MPS (qubit counts
+    log-uniform in [12, 96], caps log-uniform in [16, 256]) running the RY/RZ +
+    CZ-ladder ansatz, advanced together by grouped persistent launches
+    (``BatchedMPS``), then sampled with the grouped sweep captured in a HIP
+    graph.  Reference: the same circuits one MPS at a time with the plain
+    AdaptiveMPS method, on a subset, as updates/s."""
+    import numpy as np
+    from atlas_q.adaptive_mps import AdaptiveMPS
+    from atlas_q.batched import BatchedMPS
+    from atlas_q.circuits import batched_hea_population, hea
+    specs = batched_hea_population(a.batched_mps, a.seed + 21)
+    circs = [hea(n, a.batched_layers, a.seed + 100 + j) for j, (n, _) in enumerate(specs)]
+    n2q = sum(sum(1 for op in c if op[0] == "2q") for c in circs)
+    n1q = sum(sum(1 for op in c if op[0] == "1q") for c in circs)
+    BatchedMPS(specs, eps_bond=1e-8, device=dev).run_hea(circs)      # warm-up / autotune
+    torch.cuda.synchronize()
+    bm = BatchedMPS(specs, eps_bond=1e-8, device=dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    bm.run_hea(circs)
+    torch.cuda.synchronize()
+    t_circ = time.perf_counter() - t0
+    S = a.batched_shots
+    bm.prepare_sampler(S)
+    gen = torch.Generator(device=dev).manual_seed(a.seed + 22)
+    bm.uniforms.copy_(torch.rand(bm.uniforms.shape, dtype=torch.float64, device=dev,
+                                 generator=gen))
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        bm.sweep()
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        bm.sweep()
+    g.replay()
+    torch.cuda.synchronize()
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        g.replay()
+    torch.cuda.synchronize()
+    t_samp = (time.perf_counter() - t0) / reps
+    ref = None
+    if a.batched_ref > 0:
+        r_up, r_t = 0, 0.0
+        for j in range(min(a.batched_ref, len(specs))):
+            n, chi = specs[j]
+            m = AdaptiveMPS(n, bond_dim=1, eps_bond=1e-8, chi_max_per_bond=chi, device=dev)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for op in circs[j]:
+                if op[0] == "1q":
+                    m.apply_single_qubit_gate(op[1], torch.from_numpy(op[2]))
+                else:
+                    m.apply_two_site_gate(op[1], torch.from_numpy(op[2]))
+            torch.cuda.synchronize()
+            r_t += time.perf_counter() - t0
+            r_up += sum(1 for op in circs[j] if op[0] == "2q")
+        ref = {"mps": min(a.batched_ref, len(specs)), "updates_per_s": r_up / r_t,
+               "method": "plain AdaptiveMPS, one MPS and one gate at a time"}
+    chis = [int(bm.dims[j].max()) for j in range(len(specs))]
+    out = {"workload": f"{len(specs)} MPS, qubits log-uniform 12-96, caps log-uniform 16-256, "
+                       f"RY/RZ + CZ-ladder ansatz x {a.batched_layers} layers, complex64, "
+                       f"cutoff 1e-8, {S} shots each",
+           "two_site_updates": n2q, "single_qubit_gates": n1q,
+           "circuit_s": t_circ, "updates_per_s": n2q / t_circ,
+           "max_bond_reached": max(chis),
+           "eig_fallbacks": int(bm.eig_fallbacks),
+           "shots": S * len(specs), "sample_s": t_samp,
+           "shots_per_s": S * len(specs) / t_samp, "sampler": "grouped sweep, HIP-graph replay",
+           "unbatched_reference": ref,
+           "speedup_vs_unbatched": (n2q / t_circ) / ref["updates_per_s"] if ref else None}
+    return out
+
+
 def marker(torch):
     """One launch of the library's zero-fill kernel on one word: a dispatch
     that brackets a region in a kernel trace."""
@@ -353,6 +442,9 @@ def main():
     torch.cuda.set_device(dev)
     if a.sampler_only:
         print(json.dumps({"sampler": sampler_bench(torch, a, dev)}), flush=True)
+        return
+    if a.batched_only:
+        print(json.dumps({"batched": batched_bench(torch, a, dev)}), flush=True)
         return
     chi, sites = a.chi, a.sites
     base = slice_state(torch, sites, chi, a.seed, dev)
@@ -429,6 +521,8 @@ def main():
     log("low-entanglement done")
     samp = None if a.no_sampler else sampler_bench(torch, a, dev)
     log("sampler done")
+    bat = None if a.no_batched else batched_bench(torch, a, dev)
+    log("batched done")
 
     # ---------------------------------------------------------------- CPU baseline
     cpu = None
@@ -476,6 +570,7 @@ def main():
         "eigensolver": eig,
         "cpu_baseline": cpu,
         "sampler": samp,
+        "batched": bat,
     }
     print(json.dumps(line), flush=True)
 

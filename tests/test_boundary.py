@@ -92,3 +92,57 @@ def test_circuits_are_seeded_and_haar_su4():
     assert sum(op[0] == "2q" for op in ops) == 2 * 4
     pop = C.batched_hea_population(256, 0)
     assert len(pop) == 256 and min(p[0] for p in pop) >= 12 and max(p[0] for p in pop) <= 96
+
+
+def test_bucket_padding_keeps_gram_eigenpairs():
+    """The padding diagonal of a size bucket leaves the Gram block's
+    eigenpairs unchanged and sorts every padding eigenvalue below them,
+    with no two padding eigenvalues equal (host-side torch logic)."""
+    from atlas_q.batched import _condition_bucket
+    rng = np.random.default_rng(4)
+    sz, orders = 8, np.array([5, 8, 3])
+    T = torch.zeros((3, sz, sz), dtype=torch.complex128)
+    grams = []
+    for b, r in enumerate(orders):
+        X = rng.standard_normal((r, 2)) + 1j * rng.standard_normal((r, 2))   # rank 2
+        G = X @ X.conj().T
+        grams.append(G)
+        T[b, :r, :r] = torch.from_numpy(G)
+    _condition_bucket(T, orders, None)
+    for b, r in enumerate(orders):
+        A = T[b].numpy()
+        assert np.allclose(A[:r, :r], grams[b]) and not np.any(A[:r, r:]) and not np.any(A[r:, :r])
+        lam = np.linalg.eigvalsh(A)[::-1]
+        ref = np.linalg.eigvalsh(grams[b])[::-1]
+        assert np.allclose(lam[:r], ref, atol=1e-12)
+        pad = lam[r:]
+        assert np.all(pad < 0) and len(np.unique(pad)) == len(pad)
+        assert np.all(pad <= -np.abs(np.diag(grams[b])).max() + 1e-12)
+
+
+def test_bucket_jitter_is_small_hermitian_and_fixed():
+    from atlas_q.batched import _condition_bucket, _jitter_matrix
+    E = _jitter_matrix(16, "cpu")
+    assert torch.equal(E, _jitter_matrix(16, "cpu"))
+    assert torch.allclose(E, E.conj().T)
+    assert 0.5 < float(torch.linalg.matrix_norm(E, ord=2)) < 2.0
+    rng = np.random.default_rng(5)
+    X = rng.standard_normal((16, 3)) + 1j * rng.standard_normal((16, 3))
+    G = torch.from_numpy(X @ X.conj().T).unsqueeze(0)
+    T = G.clone()
+    _condition_bucket(T, np.array([16]), E)
+    d = float(G[0].diagonal().real.max())
+    assert torch.allclose(T, T.conj().transpose(-2, -1))
+    lam0 = np.linalg.eigvalsh(G[0].numpy())
+    lam1 = np.linalg.eigvalsh(T[0].numpy())
+    assert np.abs(lam1 - lam0).max() <= 2e-13 * d
+
+
+def test_finite_rows_flags_each_matrix():
+    from atlas_q.eig import finite_rows
+    W = torch.zeros((3, 4), dtype=torch.float64)
+    V = torch.zeros((3, 4, 4), dtype=torch.complex128)
+    W[1, 2] = float("inf")
+    V[2, 0, 3] = complex(0.0, float("nan"))
+    assert finite_rows(W, V).tolist() == [True, False, False]
+    assert finite_rows(W[:1], V[:1].conj()).tolist() == [True]

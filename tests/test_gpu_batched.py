@@ -122,3 +122,103 @@ def test_grouped_sampler_matches_single_and_graph_replays():
             assert torch.equal(eager[rep][j], bm.sample_bits(j)), (rep, j)
         # eager work between replays
         _ = (bm.arena.abs() ** 2).sum().item()
+
+
+def test_svd_fill_matches_gram_eigenpairs():
+    """Bucket rows whose eigensolve failed are refilled from the SVD of their
+    own theta: the eigen-equation of the Gram matrix holds on the non-zero
+    spectrum and the rest of the row is zero."""
+    import atlas_q.batched as BM
+    rng = np.random.default_rng(12)
+    sizes = [(6, 10), (8, 5)]                      # (m, n): left (m <= n) and right
+    xs = [rng.standard_normal(s) + 1j * rng.standard_normal(s) for s in sizes]
+    m = np.array([s[0] for s in sizes])
+    n = np.array([s[1] for s in sizes])
+    left = m <= n
+    xoff = np.array([0, m[0] * n[0]])
+    X64 = torch.from_numpy(np.concatenate([x.reshape(-1) for x in xs])).to(DEV)
+    sz = 8
+    grams = [xs[0] @ xs[0].conj().T, xs[1].conj().T @ xs[1]]
+    W = torch.full((2, sz), float("nan"), dtype=torch.float64, device=DEV)
+    V = torch.full((2, sz, sz), float("nan"), dtype=torch.complex128, device=DEV)
+    BM.BatchedMPS._svd_fill(W, V, [0, 1], np.array([0, 1]), X64, xoff, m, n, left)
+    assert bool(torch.isfinite(W).all()) and bool(torch.isfinite(torch.view_as_real(V)).all())
+    for b, G in enumerate(grams):
+        r = G.shape[0]
+        w = W[b, :r].cpu().numpy()
+        v = V[b, :r, :r].cpu().numpy()
+        assert np.allclose(np.sort(w)[::-1], np.sort(np.linalg.eigvalsh(G))[::-1], atol=1e-10)
+        k = min(xs[b].shape)                      # non-zero part of the spectrum
+        assert np.allclose(G @ v[:, :k], v[:, :k] * w[:k], atol=1e-9)
+        assert float(W[b, r:].abs().max() if r < sz else 0.0) == 0.0
+
+
+def test_batched_wide_buckets_exact_against_statevector(monkeypatch):
+    """Orders above the Jacobi limit (rocSOLVER strided-batched buckets of
+    order 128 and 256) without truncation equal the exact state; one bucket
+    row reported as failed goes through the theta-SVD refill and the state
+    is still exact."""
+    import atlas_q.batched as BM
+    specs = [(16, 256), (14, 128)]
+    circs = [hea(n, 9, 300 + j) for j, (n, _) in enumerate(specs)]
+    real = BM.heevd_batched
+    seen = {"calls": 0}
+
+    def one_failure(T, inplace=False, return_ok=False):
+        W, V, ok = real(T, inplace=inplace, return_ok=True)
+        seen["calls"] += 1
+        if seen["calls"] == 1:
+            ok = [False] + list(ok[1:])
+        return W, V, ok
+    monkeypatch.setattr(BM, "heevd_batched", one_failure)
+    bm = BatchedMPS(specs, eps_bond=1e-10, device=DEV, autotuner=_tuner())
+    bm.run_hea(circs)
+    assert seen["calls"] > 0 and bm.eig_fallbacks >= 1
+    assert max(int(bm.dims[j].max()) for j in range(len(specs))) > 64
+    for j, (n, _) in enumerate(specs):
+        psi = SV.run_circuit(n, circs[j])
+        t = bm.tensors(j)
+        res = t[0].reshape(2, -1).to(torch.complex128)
+        for q in range(1, n):
+            T = t[q].to(torch.complex128)
+            res = (res @ T.reshape(T.shape[0], -1)).reshape(-1, T.shape[2])
+        out = res.reshape(-1).cpu().numpy()
+        fid = abs(np.vdot(psi, out)) ** 2 / np.vdot(out, out).real
+        assert abs(1 - fid) < 1e-5, (j, fid)
+
+
+def test_batched_padded_buckets_match_single_mps_gram():
+    """Caps that are not powers of two put Gram matrices of order 2 cap into
+    padded buckets above the Jacobi limit (orders 80, 100, 144 in buckets
+    128 and 256); ranks and amplitudes equal the single-MPS Gram method."""
+    specs = [(12, 40), (13, 50), (14, 72)]
+    depth = 14
+    circs = [brickwork(n, depth, 70 + j) for j, (n, _) in enumerate(specs)]
+    bm = BatchedMPS(specs, eps_bond=1e-8, device=DEV, autotuner=_tuner())
+    for layer in range(depth):
+        ops = []
+        for j, (n, _) in enumerate(specs):
+            per = len(range(layer % 2, n - 1, 2))
+            start = sum(len(range(l % 2, n - 1, 2)) for l in range(layer))
+            ops.append([(b, U) for b, U in circs[j][start:start + per]])
+        bm.apply_2q_layer(ops)
+    assert bm.eig_fallbacks == 0
+    for j, (n, chi) in enumerate(specs):
+        assert int(bm.dims[j].max()) == chi
+        ref = AdaptiveMPS(n, bond_dim=1, eps_bond=1e-8, chi_max_per_bond=chi, device=DEV,
+                          svd_method="gram")
+        for b, U in circs[j]:
+            ref.apply_two_site_gate(b, torch.from_numpy(U))
+        ks = [k for mj, k in zip(bm.logs["mps"], bm.logs["k"]) if mj == j]
+        es = [e for mj, e in zip(bm.logs["mps"], bm.logs["eps_local"]) if mj == j]
+        kr, er = ref.statistics.logs["k_star"], ref.statistics.logs["eps_local"]
+        assert len(ks) == len(kr)
+        for a, b, e1, e2 in zip(ks, kr, es, er):
+            assert a == b or (e1 < 1e-3 and e2 < 1e-3)
+        for bstr in (0, 1, 2 ** n - 1, 12345 % 2 ** n):
+            bits = [(bstr >> (n - 1 - i)) & 1 for i in range(n)]
+            t = bm.tensors(j)
+            r = t[0][:, bits[0], :].to(torch.complex128)
+            for q in range(1, n):
+                r = r @ t[q][:, bits[q], :].to(torch.complex128)
+            assert abs(complex(r[0, 0]) - ref.get_amplitude(bstr)) < 1e-5

@@ -36,6 +36,39 @@ def test_heev_small_matches_lapack(n):
         assert np.allclose(np.conj(V[b].T) @ V[b], np.eye(n), atol=1e-12)
 
 
+def _kron_gram(seed):
+    rng = np.random.default_rng(seed)
+    a = rng.standard_normal((8, 4)) + 1j * rng.standard_normal((8, 4))
+    b = rng.standard_normal((8, 5)) + 1j * rng.standard_normal((8, 5))
+    X = np.kron(a, b)                                   # 64 x 20, product structure
+    return X @ X.conj().T
+
+
+def _lowrank_gram(seed):
+    rng = np.random.default_rng(seed)
+    X = rng.standard_normal((64, 24)) + 1j * rng.standard_normal((64, 24))
+    return X @ X.conj().T
+
+
+def test_heev_small_rank_deficient_spectra():
+    """Rank-deficient Gram matrices with product structure (as the Gram
+    matrices of CZ-ladder states): in the last sweeps off-diagonal entries
+    fall to ~1e-160, where an unscaled |z| = sqrt(x^2 + y^2) goes subnormal
+    and makes the rotation non-unitary (eigenvalue errors up to 1e-2 in a
+    CPU restatement of that arithmetic on these same matrices).  Eigenvalues
+    within 1e-12 of LAPACK relative to the largest, residual 1e-11."""
+    G = np.stack([_kron_gram(1), _kron_gram(2), _kron_gram(3), _lowrank_gram(3)])
+    W, V = heev_small(torch.from_numpy(G).to(DEV))
+    W, V = W.cpu().numpy(), V.cpu().numpy()
+    for b in range(len(G)):
+        ref = np.linalg.eigvalsh(G[b])[::-1]
+        scale = np.abs(ref).max()
+        assert np.abs(W[b] - ref).max() <= 1e-12 * scale, b
+        res = G[b] @ V[b] - V[b] * W[b][None, :]
+        assert np.linalg.norm(res) <= 1e-11 * np.linalg.norm(G[b]), b
+        assert np.allclose(np.conj(V[b].T) @ V[b], np.eye(64), atol=1e-12), b
+
+
 @pytest.mark.parametrize("n,batch", [(65, 3), (130, 2), (512, 4)])
 def test_heevd_batched_matches_lapack(n, batch):
     from atlas_q.eig import heevd_batched
