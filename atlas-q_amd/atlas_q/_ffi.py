@@ -87,6 +87,8 @@ SIGNATURES = {
                           c_i64, c_vp]),
     "aq_heev_small": (c_int, [c_i64, c_vp, c_i64, c_i64, c_vp, c_vp, c_i64, c_i64, c_i64, c_int,
                               ctypes.c_double, c_vp, c_vp]),
+    "aq_heev_small_orders": (c_int, [c_i64, c_vp, c_vp, c_i64, c_i64, c_vp, c_vp, c_i64, c_i64,
+                                     c_i64, c_int, ctypes.c_double, c_vp, c_vp]),
     "aq_heevd_batched": (c_int, [c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_vp, c_i64, c_vp]),
     "aq_mlp_num_params": (c_int, []),
     "aq_mlp_sync_bf16": (c_int, [c_vp, c_vp, c_vp]),

@@ -34,8 +34,9 @@ import torch
 from . import _ffi
 from ._ffi import (COPY_DESC, GEMM_DESC, SAMPLE_DESC, SITE1Q_DESC, RankResult, check, lib)
 from .autotune import Autotuner
-from .eig import JACOBI_MAX, eigh_desc, finite_rows, heevd_batched
+from .eig import JACOBI_MAX, finite_rows, heevd_batched
 from .linalg_robust import robust_svd
+from .rank_mlp import heev_small
 
 C64 = torch.complex64
 C128 = torch.complex128
@@ -258,7 +259,7 @@ class BatchedMPS:
                 _condition_bucket(T, s[idx], self._jitter[sz])
                 W, V, ok = heevd_batched(T, inplace=True, return_ok=True)
             else:
-                W, V = eigh_desc(T)
+                W, V = heev_small(T, orders=torch.from_numpy(s[idx].astype(np.int64)).to(self.dev))
                 ok = finite_rows(W, V).cpu().tolist()
             W, V = W.contiguous(), V.contiguous()
             bad = [b for b, o in enumerate(ok) if not o]

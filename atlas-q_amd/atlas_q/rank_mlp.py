@@ -50,9 +50,15 @@ def _s(t: torch.Tensor) -> ctypes.c_void_p:
     return ctypes.c_void_p(torch.cuda.current_stream(t.device).cuda_stream)
 
 
-def heev_small(G: torch.Tensor, max_sweeps: int = 30, tol: float = 1e-15):
+def heev_small(G: torch.Tensor, max_sweeps: int = 30, tol: float = 1e-15,
+               orders: torch.Tensor = None):
     """Eigen-decomposition of Hermitian complex128 matrices [batch, n, n] or
-    [n, n] (n <= 64) with the Jacobi kernel: (eigenvalues desc, eigenvectors)."""
+    [n, n] (n <= 64) with the Jacobi kernel: (eigenvalues desc, eigenvectors).
+
+    ``orders`` (int64 [batch] on the device, each <= n): matrix b is its
+    leading orders[b] block (the rest is padding, not read); its eigenvalues
+    fill W[b, :orders[b]] in descending order, W[b, orders[b]:] = 0, and the
+    padding block of V[b] is the identity."""
     single = G.dim() == 2
     if single:
         G = G.unsqueeze(0)
@@ -60,8 +66,16 @@ def heev_small(G: torch.Tensor, max_sweeps: int = 30, tol: float = 1e-15):
     bsz, n, _ = G.shape
     W = torch.empty((bsz, n), dtype=torch.float64, device=G.device)
     V = torch.empty((bsz, n, n), dtype=torch.complex128, device=G.device)
-    check(lib().aq_heev_small(n, _p(G), n, n * n, _p(W), _p(V), n, n * n, bsz, max_sweeps, tol,
-                              None, _s(G)), "aq_heev_small")
+    if orders is None:
+        check(lib().aq_heev_small(n, _p(G), n, n * n, _p(W), _p(V), n, n * n, bsz, max_sweeps,
+                                  tol, None, _s(G)), "aq_heev_small")
+    else:
+        if orders.dtype != torch.int64 or orders.device != G.device or orders.numel() != bsz:
+            raise ValueError("heev_small: orders must be int64 [batch] on the matrices' device")
+        orders = orders.contiguous()
+        check(lib().aq_heev_small_orders(n, _p(orders), _p(G), n, n * n, _p(W), _p(V), n, n * n,
+                                         bsz, max_sweeps, tol, None, _s(G)),
+              "aq_heev_small_orders")
     return (W[0], V[0]) if single else (W, V)
 
 

@@ -23,17 +23,20 @@ namespace aq {
 
 constexpr int kJmax = 64;
 
+// LDS image of one matrix of order <= JM (JM = 16, 32, 64: 13, 38, 139 KB, so
+// that orders <= 32 run several work-groups per CU; order 64 runs one
+// 512-thread work-group per CU)
+template <int JM>
 struct JacobiSmem {
-  double2 a[kJmax][kJmax + 1];
-  double2 v[kJmax][kJmax + 1];
-  double cs[kJmax / 2];       // c per pair
-  double2 sp[kJmax / 2];      // s e^{-i phi} per pair
-  double2 ep[kJmax / 2];      // e^{-i phi} per pair
-  int pp[kJmax / 2], pq[kJmax / 2];
-  int perm[kJmax];
-  double red[256];
-  double lam[kJmax];
-  int order[kJmax];
+  double2 a[JM][JM + 1];
+  double2 v[JM][JM + 1];
+  double cs[JM / 2];       // c per pair
+  double2 sp[JM / 2];      // s e^{-i phi} per pair
+  double2 ep[JM / 2];      // e^{-i phi} per pair
+  int pp[JM / 2], pq[JM / 2];
+  double red[512];
+  double lam[JM];
+  int order[JM];
 };
 
 __device__ __forceinline__ double2 cm(double2 x, double2 y) {
@@ -55,16 +58,26 @@ __device__ double block_sum(double v, double* red) {
   return r;
 }
 
-__global__ __launch_bounds__(256) void heev_jacobi_kernel(int n, const double2* __restrict__ A,
-                                                          int64_t lda, int64_t sA,
-                                                          double* __restrict__ W,
-                                                          double2* __restrict__ V, int64_t ldv,
-                                                          int64_t sV, int max_sweeps, double tol,
-                                                          int* __restrict__ sweeps_out) {
+// n: the batch's order (output row length); orders (nullable): matrix b's own
+// order nb <= n -- only its leading nb x nb block is read and diagonalised;
+// W[nb:n] = 0 and the trailing rows/columns of V are those of the identity.
+template <int JM, int NT>
+__global__ __launch_bounds__(NT) void heev_jacobi_kernel(int n, const int64_t* __restrict__ orders,
+                                                         const double2* __restrict__ A,
+                                                         int64_t lda, int64_t sA,
+                                                         double* __restrict__ W,
+                                                         double2* __restrict__ V, int64_t ldv,
+                                                         int64_t sV, int max_sweeps, double tol,
+                                                         int* __restrict__ sweeps_out) {
   extern __shared__ unsigned char smraw[];
-  JacobiSmem& sm = *reinterpret_cast<JacobiSmem*>(smraw);
+  JacobiSmem<JM>& sm = *reinterpret_cast<JacobiSmem<JM>*>(smraw);
   const int tid = threadIdx.x;
   const int64_t b = blockIdx.x;
+  const int nfull = n;
+  if (orders) {
+    const int64_t o = orders[b];
+    n = (int)(o < 1 ? 1 : (o > nfull ? nfull : o));
+  }
   const int ne = n + (n & 1);  // even order for the tournament (padding index = n)
   const double2* Ab = A + b * sA;
   for (int i = tid; i < ne * ne; i += blockDim.x) {
@@ -74,7 +87,6 @@ __global__ __launch_bounds__(256) void heev_jacobi_kernel(int n, const double2* 
     sm.a[r][c] = x;
     sm.v[r][c] = make_double2(r == c ? 1.0 : 0.0, 0.0);
   }
-  for (int i = tid; i < ne; i += blockDim.x) sm.perm[i] = i;
   __syncthreads();
   // symmetrize (the input is Hermitian up to rounding)
   for (int i = tid; i < ne * ne; i += blockDim.x) {
@@ -110,9 +122,14 @@ __global__ __launch_bounds__(256) void heev_jacobi_kernel(int n, const double2* 
     off = block_sum(off, sm.red);
     if (off <= thresh2 || ne < 2) break;
     for (int round = 0; round < ne - 1; ++round) {
-      // pairs of this round: (perm[k], perm[ne-1-k])
+      // pairs of this round: (pos(k), pos(ne-1-k)) of the round-robin
+      // tournament, position 0 fixed and the others rotated by `round`:
+      // pos(0) = 0, pos(i) = 1 + ((i - 1 - round) mod (ne - 1))
       if (tid < half) {
-        int p = sm.perm[tid], q = sm.perm[ne - 1 - tid];
+        const int m1 = ne - 1;
+        const int i2 = ne - 1 - tid;
+        int p = tid == 0 ? 0 : 1 + ((tid - 1 - round) % m1 + m1) % m1;
+        int q = 1 + ((i2 - 1 - round) % m1 + m1) % m1;
         if (p > q) { const int t = p; p = q; q = t; }
         sm.pp[tid] = p;
         sm.pq[tid] = q;
@@ -169,24 +186,19 @@ __global__ __launch_bounds__(256) void heev_jacobi_kernel(int n, const double2* 
         const double2 sec = cconj(sm.sp[k]);
         const double2 ap = sm.a[p][r], aq = sm.a[q][r];
         // new p = c ap - s conj(e) aq ; new q = s ap + c conj(e) aq
-        sm.a[p][r] = cadd2(cscale(ap, c), cscale(cm(sec, aq), -1.0));
-        sm.a[q][r] = cadd2(cscale(ap, s), cscale(cm(ec, aq), c));
-      }
-      __syncthreads();
-      // the rotated pairs are exactly diagonalised: clean the rounding
-      if (tid < half) {
-        const int p = sm.pp[tid], q = sm.pq[tid];
-        sm.a[p][q] = make_double2(0.0, 0.0);
-        sm.a[q][p] = make_double2(0.0, 0.0);
-        sm.a[p][p].y = 0.0;
-        sm.a[q][q].y = 0.0;
-      }
-      // tournament rotation: perm[0] fixed, the rest rotate by one
-      __syncthreads();
-      if (tid == 0) {
-        const int last = sm.perm[ne - 1];
-        for (int i = ne - 1; i > 1; --i) sm.perm[i] = sm.perm[i - 1];
-        sm.perm[1] = last;
+        double2 np_ = cadd2(cscale(ap, c), cscale(cm(sec, aq), -1.0));
+        double2 nq_ = cadd2(cscale(ap, s), cscale(cm(ec, aq), c));
+        // the rotated pair is exactly diagonalised: clear the rounding in
+        // its off-diagonal entries and the imaginary parts of its diagonal
+        if (r == p) {
+          np_.y = 0.0;
+          nq_ = make_double2(0.0, 0.0);
+        } else if (r == q) {
+          np_ = make_double2(0.0, 0.0);
+          nq_.y = 0.0;
+        }
+        sm.a[p][r] = np_;
+        sm.a[q][r] = nq_;
       }
       __syncthreads();
     }
@@ -204,17 +216,55 @@ __global__ __launch_bounds__(256) void heev_jacobi_kernel(int n, const double2* 
     sm.order[rank] = i;
   }
   __syncthreads();
-  double* Wb = W + b * n;
+  double* Wb = W + b * nfull;
   double2* Vb = V + b * sV;
-  for (int i = tid; i < n; i += blockDim.x) Wb[i] = sm.lam[sm.order[i]];
-  for (int i = tid; i < n * n; i += blockDim.x) {
-    const int r = i / n, c = i % n;
-    Vb[(int64_t)r * ldv + c] = sm.v[r][sm.order[c]];
+  for (int i = tid; i < nfull; i += blockDim.x) Wb[i] = i < n ? sm.lam[sm.order[i]] : 0.0;
+  for (int i = tid; i < nfull * nfull; i += blockDim.x) {
+    const int r = i / nfull, c = i % nfull;
+    double2 x = make_double2(r == c ? 1.0 : 0.0, 0.0);
+    if (r < n && c < n) x = sm.v[r][sm.order[c]];
+    Vb[(int64_t)r * ldv + c] = x;
   }
   if (tid == 0 && sweeps_out) sweeps_out[b] = sweep;
 }
 
-static bool g_jac_attr = false;
+template <int JM, int NT>
+static int launch_jacobi(int n, const int64_t* orders, const void* A, int64_t lda, int64_t sA,
+                         double* W, void* V, int64_t ldv, int64_t sV, int64_t batch,
+                         int max_sweeps, double tol, int* sweeps, hipStream_t stream) {
+  static bool attr = false;
+  if (!attr) {
+    AQ_HIP(hipFuncSetAttribute((const void*)heev_jacobi_kernel<JM, NT>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                               sizeof(JacobiSmem<JM>)));
+    attr = true;
+  }
+  heev_jacobi_kernel<JM, NT><<<(unsigned)batch, NT, sizeof(JacobiSmem<JM>), stream>>>(
+      n, orders, (const double2*)A, lda, sA, W, (double2*)V, ldv, sV, max_sweeps, tol, sweeps);
+  AQ_LAUNCH_CHECK();
+  return AQ_OK;
+}
+
+static int heev_small_impl(int64_t n, const int64_t* orders, const void* A, int64_t lda,
+                           int64_t strideA, double* W, void* V, int64_t ldv, int64_t strideV,
+                           int64_t batch, int max_sweeps, double tol, int* sweeps,
+                           hipStream_t stream) {
+  AQ_REQUIRE(A && W && V, AQ_ERR_ARG, "aq_heev_small: null pointer");
+  AQ_REQUIRE(n >= 1 && n <= kJmax, AQ_ERR_ARG, "aq_heev_small: n must be in [1, %d] (got %lld)",
+             kJmax, (long long)n);
+  AQ_REQUIRE(batch >= 1 && batch < (1LL << 31), AQ_ERR_ARG, "aq_heev_small: bad batch");
+  AQ_REQUIRE(lda >= n && ldv >= n, AQ_ERR_ARG, "aq_heev_small: leading dimension < n");
+  AQ_REQUIRE(max_sweeps >= 1 && max_sweeps <= 100, AQ_ERR_ARG, "aq_heev_small: max_sweeps");
+  // the tournament runs on the even order n + (n & 1) <= JM
+  if (n <= 16)
+    return launch_jacobi<16, 64>((int)n, orders, A, lda, strideA, W, V, ldv, strideV, batch,
+                                 max_sweeps, tol, sweeps, stream);
+  if (n <= 32)
+    return launch_jacobi<32, 256>((int)n, orders, A, lda, strideA, W, V, ldv, strideV, batch,
+                                  max_sweeps, tol, sweeps, stream);
+  return launch_jacobi<64, 512>((int)n, orders, A, lda, strideA, W, V, ldv, strideV, batch,
+                                max_sweeps, tol, sweeps, stream);
+}
 
 }  // namespace aq
 
@@ -223,20 +273,15 @@ using namespace aq;
 extern "C" int aq_heev_small(int64_t n, const void* A, int64_t lda, int64_t strideA, double* W,
                              void* V, int64_t ldv, int64_t strideV, int64_t batch, int max_sweeps,
                              double tol, int* sweeps, hipStream_t stream) {
-  AQ_REQUIRE(A && W && V, AQ_ERR_ARG, "aq_heev_small: null pointer");
-  AQ_REQUIRE(n >= 1 && n <= kJmax, AQ_ERR_ARG, "aq_heev_small: n must be in [1, %d] (got %lld)",
-             kJmax, (long long)n);
-  AQ_REQUIRE(batch >= 1 && batch < (1LL << 31), AQ_ERR_ARG, "aq_heev_small: bad batch");
-  AQ_REQUIRE(lda >= n && ldv >= n, AQ_ERR_ARG, "aq_heev_small: leading dimension < n");
-  AQ_REQUIRE(max_sweeps >= 1 && max_sweeps <= 100, AQ_ERR_ARG, "aq_heev_small: max_sweeps");
-  if (!g_jac_attr) {
-    AQ_HIP(hipFuncSetAttribute((const void*)heev_jacobi_kernel,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, sizeof(JacobiSmem)));
-    g_jac_attr = true;
-  }
-  heev_jacobi_kernel<<<(unsigned)batch, 256, sizeof(JacobiSmem), stream>>>(
-      (int)n, (const double2*)A, lda, strideA, W, (double2*)V, ldv, strideV, max_sweeps, tol,
-      sweeps);
-  AQ_LAUNCH_CHECK();
-  return AQ_OK;
+  return heev_small_impl(n, nullptr, A, lda, strideA, W, V, ldv, strideV, batch, max_sweeps, tol,
+                         sweeps, stream);
+}
+
+extern "C" int aq_heev_small_orders(int64_t n, const int64_t* orders, const void* A, int64_t lda,
+                                    int64_t strideA, double* W, void* V, int64_t ldv,
+                                    int64_t strideV, int64_t batch, int max_sweeps, double tol,
+                                    int* sweeps, hipStream_t stream) {
+  AQ_REQUIRE(orders, AQ_ERR_ARG, "aq_heev_small_orders: null orders");
+  return heev_small_impl(n, orders, A, lda, strideA, W, V, ldv, strideV, batch, max_sweeps, tol,
+                         sweeps, stream);
 }

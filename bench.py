@@ -38,6 +38,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -293,6 +294,19 @@ def sampler_bench(torch, a, dev):
     return out
 
 
+def mps_fidelity(torch, ta, tb) -> float:
+    """|<a|b>|^2 / (<a|a> <b|b>) of two MPS on the device (complex128
+    transfer matrices, one site at a time)."""
+    def overlap(x, y):
+        E = torch.ones((1, 1), dtype=torch.complex128, device=x[0].device)
+        for A, B in zip(x, y):
+            A, B = A.to(torch.complex128), B.to(torch.complex128)
+            E = sum(A[:, s, :].conj().T @ E @ B[:, s, :] for s in range(2))
+        return complex(E.reshape(-1)[0])
+    ab = overlap(ta, tb)
+    return abs(ab) ** 2 / (overlap(ta, ta).real * overlap(tb, tb).real)
+
+
 def batched_bench(torch, a, dev):
     """The batched form: a population of independent MPS (qubit counts
     log-uniform in [12, 96], caps log-uniform in [16, 256]) running the RY/RZ +
@@ -339,6 +353,7 @@ def batched_bench(torch, a, dev):
     torch.cuda.synchronize()
     t_samp = (time.perf_counter() - t0) / reps
     ref = None
+    fids = []
     if a.batched_ref > 0:
         r_up, r_t = 0, 0.0
         for j in range(min(a.batched_ref, len(specs))):
@@ -354,8 +369,25 @@ def batched_bench(torch, a, dev):
             torch.cuda.synchronize()
             r_t += time.perf_counter() - t0
             r_up += sum(1 for op in circs[j] if op[0] == "2q")
+            eb = math.sqrt(sum(e * e for mj, e in zip(bm.logs["mps"], bm.logs["eps_local"])
+                               if mj == j))
+            ep = math.sqrt(sum(e * e for e in m.statistics.logs["eps_local"]))
+            fids.append({"qubits": n, "cap": chi, "fidelity": mps_fidelity(torch, bm.tensors(j),
+                                                                          m.tensors),
+                         "error_bound_batched": eb, "error_bound_plain": ep})
+        light = [f["fidelity"] for f in fids if f["error_bound_plain"] < 1e-3]
+        heavy = [abs(f["error_bound_batched"] - f["error_bound_plain"]) / f["error_bound_plain"]
+                 for f in fids if f["error_bound_plain"] >= 1e-3]
         ref = {"mps": min(a.batched_ref, len(specs)), "updates_per_s": r_up / r_t,
-               "method": "plain AdaptiveMPS, one MPS and one gate at a time"}
+               "method": "plain AdaptiveMPS, one MPS and one gate at a time",
+               "per_mps": fids,
+               "min_fidelity_where_truncation_negligible": min(light) if light else None,
+               "max_rel_diff_error_bound_where_caps_bind": max(heavy) if heavy else None,
+               "note": "where caps cut through the exactly degenerate Schmidt spectra of these "
+                       "circuits, any two SVD implementations keep different (equally valid) "
+                       "subspaces; the reference run in complex64 and in complex128 differs "
+                       "there too, so the truncated states are compared by their global error "
+                       "bound sqrt(sum eps_local^2)"}
     chis = [int(bm.dims[j].max()) for j in range(len(specs))]
     out = {"workload": f"{len(specs)} MPS, qubits log-uniform 12-96, caps log-uniform 16-256, "
                        f"RY/RZ + CZ-ladder ansatz x {a.batched_layers} layers, complex64, "

@@ -154,6 +154,15 @@ int aq_sample(int dtype, int prec, int flags, int64_t n, const void* const* site
 int aq_heev_small(int64_t n, const void* A, int64_t lda, int64_t strideA, double* W, void* V,
                   int64_t ldv, int64_t strideV, int64_t batch, int max_sweeps, double tol,
                   int* sweeps, hipStream_t stream);
+/* aq_heev_small with a per-matrix order (no reference counterpart; used for
+ * size buckets of Gram matrices of different orders): orders (device,
+ * int64[batch]) holds matrix b's order nb <= n; only the leading nb x nb block
+ * is read and diagonalised, W[b, nb:n] = 0 and the trailing rows/columns of
+ * V's n x n block are those of the identity. */
+int aq_heev_small_orders(int64_t n, const int64_t* orders, const void* A, int64_t lda,
+                         int64_t strideA, double* W, void* V, int64_t ldv, int64_t strideV,
+                         int64_t batch, int max_sweeps, double tol, int* sweeps,
+                         hipStream_t stream);
 
 /* Batched FP64 Hermitian eigen-decomposition of the truncated SVD's Gram
  * cores (no reference counterpart: the reference factors theta with a thin

@@ -69,6 +69,33 @@ def test_heev_small_rank_deficient_spectra():
         assert np.allclose(np.conj(V[b].T) @ V[b], np.eye(64), atol=1e-12), b
 
 
+@pytest.mark.parametrize("n", [8, 16, 32, 64])
+def test_heev_small_per_matrix_orders(n):
+    """Each matrix of a padded bucket is diagonalised at its own order: the
+    leading block's eigenpairs match LAPACK, W is zero and V the identity on
+    the padding (whose contents are not read)."""
+    rng = np.random.default_rng(100 + n)
+    orders = np.array([n, max(1, n // 2 + 1), 1, max(2, n - 3), max(1, n // 4)])
+    G = np.full((len(orders), n, n), np.nan + 0j)            # padding must not be read
+    for b, o in enumerate(orders):
+        Z = rng.standard_normal((o, o)) + 1j * rng.standard_normal((o, o))
+        G[b, :o, :o] = Z @ Z.conj().T
+    G[1, :orders[1], :orders[1]] = _kron_gram(1)[:orders[1], :orders[1]]
+    W, V = heev_small(torch.from_numpy(G).to(DEV),
+                      orders=torch.from_numpy(orders.astype(np.int64)).to(DEV))
+    W, V = W.cpu().numpy(), V.cpu().numpy()
+    for b, o in enumerate(orders):
+        blk = G[b, :o, :o]
+        ref = np.linalg.eigvalsh(blk)[::-1]
+        scale = max(np.abs(ref).max(), 1e-300)
+        assert np.abs(W[b, :o] - ref).max() <= 1e-12 * scale, (b, o)
+        assert np.linalg.norm(blk @ V[b, :o, :o] - V[b, :o, :o] * W[b, :o]) <= 1e-11 * np.linalg.norm(blk)
+        assert np.all(W[b, o:] == 0)
+        pad = V[b].copy()
+        pad[:o, :o] = np.eye(o)
+        assert np.array_equal(pad, np.eye(n))
+
+
 @pytest.mark.parametrize("n,batch", [(65, 3), (130, 2), (512, 4)])
 def test_heevd_batched_matches_lapack(n, batch):
     from atlas_q.eig import heevd_batched
