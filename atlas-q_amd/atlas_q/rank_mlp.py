@@ -87,6 +87,9 @@ def gate_schmidt(gate64: np.ndarray) -> np.ndarray:
     return s / max(np.linalg.norm(s), 1e-300)
 
 
+CENSOR_BAND = 0.25     # log2 units
+
+
 class RankPredictor:
     """Device-resident MLP with feature standardisation."""
 
@@ -98,6 +101,10 @@ class RankPredictor:
         self.mean = torch.zeros(NFEAT, dtype=torch.float32, device=self.device)
         self.std = torch.ones(NFEAT, dtype=torch.float32, device=self.device)
         self.margin = 1.15          # width multiplier on the predicted rank
+        # largest log2(kept rank) of the training targets: a prediction within
+        # CENSOR_BAND of it is censored (the rank may be larger than anything
+        # seen in training) and read as "at least the cap"
+        self.log2_max = 20.0
         self._sync()
         self._gen = torch.Generator(device=self.device)
         self._gen.manual_seed(777)
@@ -129,7 +136,8 @@ class RankPredictor:
 
     def save(self, path):
         np.savez(path, master=self.master.cpu().numpy(), mean=self.mean.cpu().numpy(),
-                 std=self.std.cpu().numpy(), margin=np.float64(self.margin))
+                 std=self.std.cpu().numpy(), margin=np.float64(self.margin),
+                 log2_max=np.float64(self.log2_max))
 
     @classmethod
     def load(cls, path=DEFAULT_WEIGHTS, device="cuda"):
@@ -139,6 +147,8 @@ class RankPredictor:
         r.mean = torch.from_numpy(d["mean"].astype(np.float32)).to(r.device)
         r.std = torch.from_numpy(d["std"].astype(np.float32)).to(r.device)
         r.margin = float(d["margin"])
+        # weights saved before the field existed were trained on kept ranks <= 1024
+        r.log2_max = float(d["log2_max"]) if "log2_max" in d.files else 10.0
         r._sync()
         r.trained = True
         return r
@@ -183,6 +193,8 @@ class RankPredictor:
 
     def predict_width(self, feats: torch.Tensor, cap: int, rmax: int) -> int:
         y = float(self.predict_log2(feats.unsqueeze(0))[0].item())
+        if y >= self.log2_max - CENSOR_BAND:
+            return max(1, min(cap, rmax))      # at the edge of the training range
         k = int(math.ceil(self.margin * 2.0 ** max(0.0, min(y, 20.0))))
         return max(1, min(k, cap, rmax))
 
@@ -192,6 +204,7 @@ class RankPredictor:
         """AdamW on MSE(log2 k) with the library's kernels; returns the loss log."""
         F = np.asarray(F, np.float32)
         t = np.log2(np.asarray(k, np.float64)).astype(np.float32)
+        self.log2_max = float(t.max())
         mean = F.mean(0)
         std = F.std(0) + 1e-3
         self.mean = torch.from_numpy(mean).to(self.device)

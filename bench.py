@@ -93,6 +93,13 @@ def parse():
                     help="MPS of the population run one by one (plain method) as the "
                          "unbatched reference rate")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--wide-chi", type=int, default=2048,
+                    help="bond of the learned-rank form's slice (chi_max 2048)")
+    ap.add_argument("--wide-sites", type=int, default=12)
+    ap.add_argument("--wide-plain-steps", type=int, default=2)
+    ap.add_argument("--wide-fidelity", type=int, default=1)
+    ap.add_argument("--no-wide", action="store_true")
+    ap.add_argument("--wide-only", action="store_true")
     ap.add_argument("--batched-only", action="store_true")
     ap.add_argument("--mark", action="store_true",
                     help="launch a one-element zero-fill kernel right before and after the "
@@ -294,6 +301,51 @@ def sampler_bench(torch, a, dev):
     return out
 
 
+def wide_bench(torch, a, dev):
+    """The learned-rank form's bond (chi_max 2048, theta 4096 x 4096): a
+    steady-state slice of WIDE_SITES sites at bond 2048 (the 256-qubit
+    depth-80 brickwork's saturated bulk), one sub-layer of warm-up, then one
+    odd and one even sub-layer timed; the plain GPU SVD path on a few of the
+    same gates; truncation fidelity of both against the FP64 truncation."""
+    from atlas_q.adaptive_mps import _gate4
+    chi, sites = a.wide_chi, a.wide_sites
+    base = slice_state(torch, sites, chi, a.seed + 31, dev)
+    sched = gate_schedule(sites, 3 * sites, a.seed + 32)
+    runs = sublayers(sched)
+    warm, timed = runs[0], runs[1] + runs[2]
+    rest = [g for r in runs[3:] for g in r]
+    fast = make_mps(torch, base, chi, a.method, a.theta_precision)
+    run_gates(torch, fast, warm, a.workers)
+    t = run_gates(torch, fast, timed, a.workers)
+    out = {"workload": f"steady-state slice of the 256-qubit depth-80 Haar brickwork at "
+                       f"chi_max={chi}: {sites} sites, all bonds {chi}, cutoff 1e-8, complex64; "
+                       f"theta {2 * chi}x{2 * chi}",
+           "updates": len(timed), "updates_per_s": len(timed) / t,
+           "kept_ranks": sorted(set(fast.statistics.logs["k_star"][len(warm):])),
+           "factor_counts": dict(fast.factor_stats.counts)}
+    plain = None
+    if not a.no_plain and a.wide_plain_steps > 0:
+        pm = make_mps(torch, fast.tensors, chi, "plain", "native")
+        run_gates(torch, pm, rest[:1], 1)                                   # library warm-up
+        plain = a.wide_plain_steps / run_gates(torch, pm, rest[1: 1 + a.wide_plain_steps],
+                                               a.workers)
+    out["plain_updates_per_s"] = plain
+    out["speedup_vs_plain"] = out["updates_per_s"] / plain if plain else None
+    fid = []
+    for b, U in rest[1 + a.wide_plain_steps: 1 + a.wide_plain_steps + a.wide_fidelity]:
+        outs = {}
+        for name, meth, prec in (("fast", a.method, a.theta_precision), ("plain", "plain", "native")):
+            mm = make_mps(torch, fast.tensors, chi, meth, prec)
+            mm.apply_two_site_gate(b, U)
+            outs[name] = (mm.tensors[b], mm.tensors[b + 1])
+        fid.append(fidelity_vs_fp64(torch, fast.tensors[b], fast.tensors[b + 1], _gate4(U), chi,
+                                    outs))
+    out["fidelity"] = fid
+    out["max_abs_dF_fast_vs_plain"] = max((abs(f["fast"]["F"] - f["plain"]["F"]) for f in fid),
+                                         default=None)
+    return out
+
+
 def mps_fidelity(torch, ta, tb) -> float:
     """|<a|b>|^2 / (<a|a> <b|b>) of two MPS on the device (complex128
     transfer matrices, one site at a time)."""
@@ -478,6 +530,9 @@ def main():
     if a.batched_only:
         print(json.dumps({"batched": batched_bench(torch, a, dev)}), flush=True)
         return
+    if a.wide_only:
+        print(json.dumps({"learned_rank_form": wide_bench(torch, a, dev)}), flush=True)
+        return
     chi, sites = a.chi, a.sites
     base = slice_state(torch, sites, chi, a.seed, dev)
     total = a.warmup + a.steps
@@ -555,6 +610,8 @@ def main():
     log("sampler done")
     bat = None if a.no_batched else batched_bench(torch, a, dev)
     log("batched done")
+    wide = None if a.no_wide else wide_bench(torch, a, dev)
+    log("learned-rank form (chi 2048) done")
 
     # ---------------------------------------------------------------- CPU baseline
     cpu = None
@@ -603,6 +660,7 @@ def main():
         "cpu_baseline": cpu,
         "sampler": samp,
         "batched": bat,
+        "learned_rank_form": wide,
     }
     print(json.dumps(line), flush=True)
 

@@ -146,3 +146,18 @@ def test_finite_rows_flags_each_matrix():
     V[2, 0, 3] = complex(0.0, float("nan"))
     assert finite_rows(W, V).tolist() == [True, False, False]
     assert finite_rows(W[:1], V[:1].conj()).tolist() == [True]
+
+
+def test_rank_width_censored_at_training_edge():
+    """A predicted log2(rank) within 0.25 of the largest training target is
+    read as "at least the cap" (the network never saw larger ranks); below
+    the edge the width is ceil(margin 2^y), clipped to the cap."""
+    from atlas_q import rank_mlp
+    p = object.__new__(rank_mlp.RankPredictor)
+    p.margin, p.log2_max = 1.15, 10.0
+    for y, cap, want in ((9.9, 2048, 2048), (9.75, 2048, 2048), (9.7, 2048, 957),
+                         (5.0, 2048, 37), (12.0, 512, 512)):
+        p.predict_log2 = lambda f, y=y: torch.tensor([y])
+        assert p.predict_width(torch.zeros(32), cap, 4096) == want, (y, cap)
+    d = np.load(rank_mlp.DEFAULT_WEIGHTS, allow_pickle=False)
+    assert "log2_max" not in d.files or float(d["log2_max"]) >= 10.0
