@@ -332,14 +332,19 @@ def wide_bench(torch, a, dev):
     out["plain_updates_per_s"] = plain
     out["speedup_vs_plain"] = out["updates_per_s"] / plain if plain else None
     fid = []
-    for b, U in rest[1 + a.wide_plain_steps: 1 + a.wide_plain_steps + a.wide_fidelity]:
-        outs = {}
-        for name, meth, prec in (("fast", a.method, a.theta_precision), ("plain", "plain", "native")):
-            mm = make_mps(torch, fast.tensors, chi, meth, prec)
-            mm.apply_two_site_gate(b, U)
-            outs[name] = (mm.tensors[b], mm.tensors[b + 1])
-        fid.append(fidelity_vs_fp64(torch, fast.tensors[b], fast.tensors[b + 1], _gate4(U), chi,
-                                    outs))
+    if a.wide_fidelity > 0:
+        # the fast path as timed (a whole sub-layer, grouped); plain gate by gate
+        sub = runs[4]
+        fm = make_mps(torch, fast.tensors, chi, a.method, a.theta_precision)
+        fm.apply_two_site_gates(sub, workers=a.workers)
+        out["fidelity_sublayer_factor_counts"] = dict(fm.factor_stats.counts)
+        for b, U in sub[: a.wide_fidelity]:
+            pm = make_mps(torch, fast.tensors, chi, "plain", "native")
+            pm.apply_two_site_gate(b, U)
+            outs = {"fast": (fm.tensors[b], fm.tensors[b + 1]),
+                    "plain": (pm.tensors[b], pm.tensors[b + 1])}
+            fid.append(fidelity_vs_fp64(torch, fast.tensors[b], fast.tensors[b + 1], _gate4(U),
+                                        chi, outs))
     out["fidelity"] = fid
     out["max_abs_dF_fast_vs_plain"] = max((abs(f["fast"]["F"] - f["plain"]["F"]) for f in fid),
                                          default=None)
@@ -536,8 +541,7 @@ def main():
     chi, sites = a.chi, a.sites
     base = slice_state(torch, sites, chi, a.seed, dev)
     total = a.warmup + a.steps
-    sched = gate_schedule(sites, total + max(a.plain_steps, a.fidelity_samples, a.seq_steps),
-                          a.seed + 1)
+    sched = gate_schedule(sites, total + max(a.plain_steps, a.seq_steps, sites), a.seed + 1)
 
     log(f"slice ready: {sites} sites at bond {chi}")
 
@@ -559,15 +563,20 @@ def main():
     if roof and chi == 1024 and a.sites == 128:
         roof["traffic"], roof["traffic_source"] = traffic_from_profile(roof["kernel"])
     eig = None
-    if roof and "eig_zheevd_batched" in roof["all"]:
-        e = roof["all"]["eig_zheevd_batched"]
-        eig = {"what": "rocSOLVER strided-batched zheevd over the FP64 Gram cores of a sub-layer, "
-                       "HIP events around each call; bytes = n^3/3 complex128 per matrix, the "
-                       "trailing blocks its one-stage tridiagonalisation streams",
-               "calls": e["launches"], "mean_ms_per_call": e["mean_ms"],
-               "ms_per_matrix": e["total_ms"] / a.steps,
-               "share_of_timed_region": e["total_ms"] / (1e3 * t_fast),
-               "streamed_GBps": e["achieved"], "frac_of_hbm_peak": e["frac"]}
+    if roof:
+        eig = {}
+        for key, what in (("eig_zheevd_batched", "rocSOLVER strided-batched zheevd over the FP64 "
+                                                 "Gram cores of a sub-layer; bytes = n^3/3 "
+                                                 "complex128 per matrix"),):
+            if key in roof["all"]:
+                e = roof["all"][key]
+                eig[key] = {"what": what + " (the trailing blocks a one-stage "
+                                           "tridiagonalisation streams); HIP events around each call",
+                            "calls": e["launches"], "mean_ms_per_call": e["mean_ms"],
+                            "total_ms": e["total_ms"],
+                            "share_of_timed_region": e["total_ms"] / (1e3 * t_fast),
+                            "streamed_GBps": e["achieved"], "frac_of_hbm_peak": e["frac"]}
+        eig["ms_per_update"] = sum(v["total_ms"] for v in eig.values()) / a.steps
     ks = fast.statistics.logs["k_star"][a.warmup:]
     fast_seq = None
     if a.workers > 1 and a.seq_steps > 0:
@@ -590,17 +599,23 @@ def main():
     log(f"plain path: {plain} updates/s")
     # ---------------------------------------------------------------- fidelity
     fid = []
+    fid_counts = None
     if a.fidelity_samples > 0:
         from atlas_q.adaptive_mps import _gate4
         state = fast.tensors
-        for b, U in sched[total: total + a.fidelity_samples]:
-            g = _gate4(U)
-            outs = {}
-            for name, meth, prec in (("fast", a.method, a.theta_precision), ("plain", "plain", "native")):
-                mm = make_mps(torch, state, chi, meth, prec)
-                mm.apply_two_site_gate(b, U)
-                outs[name] = (mm.tensors[b], mm.tensors[b + 1])
-            fid.append(fidelity_vs_fp64(torch, state[b], state[b + 1], g, chi, outs))
+        # the fast path as timed: the whole next sub-layer through the grouped
+        # factorization; the plain path gate by gate (no grouping there)
+        sub = sublayers(sched[total: total + sites])[0]
+        fm = make_mps(torch, state, chi, a.method, a.theta_precision)
+        fm.apply_two_site_gates(sub, workers=a.workers)
+        fid_counts = dict(fm.factor_stats.counts)
+        step = max(1, len(sub) // a.fidelity_samples)
+        for b, U in sub[::step][: a.fidelity_samples]:
+            pm = make_mps(torch, state, chi, "plain", "native")
+            pm.apply_two_site_gate(b, U)
+            outs = {"fast": (fm.tensors[b], fm.tensors[b + 1]),
+                    "plain": (pm.tensors[b], pm.tensors[b + 1])}
+            fid.append(fidelity_vs_fp64(torch, state[b], state[b + 1], _gate4(U), chi, outs))
     dF = max((abs(f["fast"]["F"] - f["plain"]["F"]) for f in fid), default=None)
 
     log("fidelity done")
@@ -651,7 +666,8 @@ def main():
         "sequential": {"fast_updates_per_s": fast_seq, "plain_updates_per_s": plain_seq,
                        "speedup": (fast_seq / plain_seq) if fast_seq and plain_seq else None},
         "kept_ranks": sorted(set(ks)),
-        "fidelity": {"max_abs_dF_fast_vs_plain": dF, "samples": fid},
+        "fidelity": {"max_abs_dF_fast_vs_plain": dF, "samples": fid,
+                     "fast_sublayer_factor_counts": fid_counts},
         "factor_counts": fast.factor_stats.counts if hasattr(fast, "factor_stats") else None,
         "factor_rejects": fast.factor_stats.rejects if hasattr(fast, "factor_stats") else None,
         "low_entanglement": low,
