@@ -253,3 +253,32 @@ def test_sampler_medium_size_against_oracle():
     got = np.array(K.bits_to_ints(bits))
     assert ok.mean() > 0.9
     assert np.array_equal(got[ok], np.array(ref)[ok])
+
+
+def test_runner_ghz_counts_small_and_beyond_64_qubits():
+    """The gate-list runner (qiskit_adapter.py:383-405).  The reference's
+    sample() dispatch samples the chain as it stands above 10 qubits (and
+    left-canonicalises at <= 10), while the sweep assumes right-canonical
+    sites, so its counts follow the Born rule only on right-canonical chains;
+    the runner keeps that by default (totals and seeded determinism checked)
+    and with ``born_exact=True`` right-canonicalises first: GHZ circuits then
+    give only the all-zeros and all-ones keys (qubit 0 leftmost) with
+    frequencies near 1/2, also beyond 64 qubits where the reference's int
+    keys break.  n <= 20 also returns the normalised state vector."""
+    from atlas_q.runner import run_mps
+    for n, exact in ((5, False), (5, True), (70, False), (70, True)):
+        gates = [("h", [0], [])] + [("cx", [q, q + 1], []) for q in range(n - 1)]
+        counts, sv = run_mps(gates, n, 400, seed=11, chi_max_per_bond=8, born_exact=exact)
+        assert sum(counts.values()) == 400 and all(len(k) == n for k in counts)
+        if exact:
+            assert set(counts) <= {"0" * n, "1" * n}
+            assert min(counts.values()) > 140
+        again, _ = run_mps(gates, n, 400, seed=11, chi_max_per_bond=8, born_exact=exact)
+        assert again == counts
+        if n <= 20:
+            v = sv.detach().cpu().numpy().reshape(-1)
+            assert abs(np.vdot(v, v) - 1) < 1e-5 and abs(abs(v[0]) ** 2 - 0.5) < 1e-5
+        else:
+            assert sv is None
+    counts, _ = run_mps([("x", [0], []), ("ry", [2], [np.pi])], 3, 50, seed=1, born_exact=True)
+    assert counts == {"101": 50}
