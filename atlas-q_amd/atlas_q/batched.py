@@ -50,8 +50,14 @@ def _stream() -> ctypes.c_void_p:
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
+def _h2d(arr: np.ndarray, device) -> torch.Tensor:
+    """Host array -> device tensor (a synchronous copy of a private host copy)."""
+    return torch.from_numpy(np.ascontiguousarray(arr).copy()).to(device)
+
+
 def _upload(arr: np.ndarray, device) -> torch.Tensor:
-    return torch.from_numpy(np.ascontiguousarray(arr).view(np.uint8).copy()).to(device)
+    """A structured array's bytes on the device."""
+    return _h2d(np.ascontiguousarray(arr).view(np.uint8), device)
 
 
 def _prefix(counts: np.ndarray) -> Tuple[np.ndarray, int]:
@@ -63,6 +69,36 @@ def _prefix(counts: np.ndarray) -> Tuple[np.ndarray, int]:
 
 def _cdiv(a, b):
     return -(-a // b)
+
+
+def _merge_per_site(jj: np.ndarray, qq: np.ndarray, U: np.ndarray, nmax: int):
+    """Gates U[g] on sites (jj[g], qq[g]) in op order, merged per site into one
+    matrix per site: U_first @ ... @ U_last.  Returns (jj, qq, U) per site."""
+    key = jj * (nmax + 1) + qq
+    order = np.argsort(key, kind="stable")
+    ks = key[order]
+    first = np.ones(len(ks), bool)
+    first[1:] = ks[1:] != ks[:-1]
+    gid = np.cumsum(first) - 1
+    heads = np.flatnonzero(first)
+    rank = np.arange(len(ks)) - heads[gid]
+    Us = U[order]
+    M = Us[heads].copy()
+    for r in range(1, int(rank.max()) + 1):
+        sel = rank == r
+        M[gid[sel]] = M[gid[sel]] @ Us[sel]
+    return jj[order[heads]], qq[order[heads]], M
+
+
+def _ragged_arange(lens: np.ndarray) -> np.ndarray:
+    """concatenate([arange(L) for L in lens]) without a Python loop."""
+    lens = np.asarray(lens, np.int64)
+    total = int(lens.sum())
+    if total == 0:
+        return np.zeros(0, np.int64)
+    starts = np.zeros(len(lens), np.int64)
+    starts[1:] = np.cumsum(lens[:-1])
+    return np.arange(total, dtype=np.int64) - np.repeat(starts, lens)
 
 
 def _condition_bucket(T: torch.Tensor, orders: np.ndarray, jitter: torch.Tensor):
@@ -152,23 +188,35 @@ class BatchedMPS:
     def _ptr(self, j, q) -> int:
         return self._base + 8 * int(self.slot_off[j, q])
 
+    def _ptrs(self, jj: np.ndarray, qq: np.ndarray) -> np.ndarray:
+        """Slot addresses of sites (jj[g], qq[g]) (vectorised ``_ptr``)."""
+        return self._base + 8 * self.slot_off[jj, qq]
+
     # ------------------------------------------------------------ 1q layer
     def apply_1q_layer(self, ops: Sequence[Sequence[Tuple[int, np.ndarray]]]):
         """ops[j] = [(q, U2), ...]; at most one gate per site per call."""
         rows = [(j, q, U) for j, lst in enumerate(ops) for q, U in lst]
         if not rows:
             return
-        d = np.zeros(len(rows), SITE1Q_DESC)
-        for i, (j, q, U) in enumerate(rows):
-            p = self._ptr(j, q)
-            d[i]["T_in"] = p
-            d[i]["T_out"] = p
-            d[i]["chiL"] = self.dims[j, q]
-            d[i]["chiR"] = self.dims[j, q + 1]
-            d[i]["gate"] = np.asarray(U, np.complex128).reshape(-1).view(np.float64)
+        self.apply_1q_arrays(np.array([r[0] for r in rows], np.int64),
+                             np.array([r[1] for r in rows], np.int64),
+                             np.stack([np.asarray(r[2], np.complex128).reshape(2, 2) for r in rows]))
+
+    def apply_1q_arrays(self, jj: np.ndarray, qq: np.ndarray, U: np.ndarray):
+        """Gate U[g] (2 x 2, U^T convention) on site qq[g] of MPS jj[g]; at most
+        one gate per site per call."""
+        if len(jj) == 0:
+            return
+        d = np.zeros(len(jj), SITE1Q_DESC)
+        p = self._ptrs(jj, qq)
+        d["T_in"] = p
+        d["T_out"] = p
+        d["chiL"] = self.dims[jj, qq]
+        d["chiR"] = self.dims[jj, qq + 1]
+        d["gate"] = np.ascontiguousarray(U, np.complex128).reshape(len(jj), 4).view(np.float64)
         cnt = d["chiL"] * d["chiR"]
         pre, total = _prefix(cnt)
-        dd, pd = _upload(d, self.dev), torch.from_numpy(pre).to(self.dev)
+        dd, pd = _upload(d, self.dev), _h2d(pre, self.dev)
         grid = self.tuner.grid("elem", total, None)   # in place: no timing trials
         check(lib().aq_grouped_1q(_vp(dd), _vp(pd), len(d), total, grid, _stream()),
               "aq_grouped_1q")
@@ -181,7 +229,7 @@ class BatchedMPS:
         tn = np.where(paired, _cdiv(descs["N"] // 2, 64), _cdiv(descs["N"], 128))
         tiles = _cdiv(descs["M"], 64) * tn
         pre, total = _prefix(tiles)
-        dd, pd = _upload(descs, self.dev), torch.from_numpy(pre).to(self.dev)
+        dd, pd = _upload(descs, self.dev), _h2d(pre, self.dev)
 
         def run(g):
             check(lib().aq_grouped_gemm(dtype_code, _vp(dd), _vp(pd), len(descs), total, layout,
@@ -190,13 +238,20 @@ class BatchedMPS:
 
     def apply_2q_layer(self, ops: Sequence[Sequence[Tuple[int, np.ndarray]]]):
         """ops[j] = [(bond, U4), ...] with disjoint bonds within each MPS."""
-        gates = [(j, b, np.asarray(U, np.complex128).reshape(4, 4))
-                 for j, lst in enumerate(ops) for b, U in lst]
-        G = len(gates)
+        gates = [(j, b, U) for j, lst in enumerate(ops) for b, U in lst]
+        if not gates:
+            return
+        self.apply_2q_arrays(np.array([g[0] for g in gates], np.int64),
+                             np.array([g[1] for g in gates], np.int64),
+                             np.stack([np.asarray(g[2], np.complex128).reshape(4, 4)
+                                       for g in gates]))
+
+    def apply_2q_arrays(self, jj: np.ndarray, bb: np.ndarray, U4: np.ndarray):
+        """Gate U4[g] (4 x 4, "U" convention) on bond bb[g] of MPS jj[g]; the
+        bonds of one MPS must be disjoint."""
+        G = len(jj)
         if G == 0:
             return
-        jj = np.array([g[0] for g in gates], np.int64)
-        bb = np.array([g[1] for g in gates], np.int64)
         dl = self.dims[jj, bb]
         dm = self.dims[jj, bb + 1]
         dr = self.dims[jj, bb + 2]
@@ -205,15 +260,15 @@ class BatchedMPS:
         xoff, xtot = _prefix(m * n)
         X = torch.empty(xtot, dtype=C64, device=self.dev)
         th = np.zeros(G, GEMM_DESC)
-        th["A"] = [self._ptr(j, b) for j, b in zip(jj, bb)]
-        th["B"] = [self._ptr(j, b + 1) for j, b in zip(jj, bb)]
+        th["A"] = self._ptrs(jj, bb)
+        th["B"] = self._ptrs(jj, bb + 1)
         th["C"] = X.data_ptr() + 8 * xoff
         th["M"], th["N"], th["K"] = m, n, dm
         th["a_s0"], th["a_s1"] = dm, 1
         th["b_s0"], th["b_s1"] = n, 1
         th["c_s0"], th["c_s1"] = n, 1
         th["kind"] = _ffi.AQ_GROUPED_THETA
-        th["gate"] = np.stack([g[2].reshape(-1).view(np.float64) for g in gates])
+        th["gate"] = np.ascontiguousarray(U4, np.complex128).reshape(G, 16).view(np.float64)
         self._grouped_gemm(_ffi.AQ_C64, th, 3, "theta")
         X64 = X.to(C128)
         x64 = X64.data_ptr() + 16 * xoff
@@ -250,46 +305,33 @@ class BatchedMPS:
         gram["kind"] = _ffi.AQ_GROUPED_GEMM
         self._grouped_gemm(_ffi.AQ_C128, gram[left], 1 | 4, "gram")
         self._grouped_gemm(_ffi.AQ_C128, gram[~left], 2 | 4, "gram")
-        # ---- eigen-decompositions per bucket (descending)
-        W_all, V_all = {}, {}
+        # ---- eigen-decompositions per bucket (descending); success flags stay
+        # on the device and come back with the rank results (one sync)
+        W_all, V_all, oks = {}, {}, []
         for sz, (idx, T) in buckets.items():
             if sz > JACOBI_MAX:
                 if sz not in self._jitter:
                     self._jitter[sz] = _jitter_matrix(sz, self.dev)
                 _condition_bucket(T, s[idx], self._jitter[sz])
-                W, V, ok = heevd_batched(T, inplace=True, return_ok=True)
+                W, V, ok = heevd_batched(T, inplace=True, return_ok="device")
             else:
-                W, V = heev_small(T, orders=torch.from_numpy(s[idx].astype(np.int64)).to(self.dev))
-                ok = finite_rows(W, V).cpu().tolist()
-            W, V = W.contiguous(), V.contiguous()
-            bad = [b for b, o in enumerate(ok) if not o]
-            if bad:
-                self.eig_fallbacks += len(bad)
-                self._svd_fill(W, V, bad, idx, X64, xoff, m, n, left)
-            W_all[sz], V_all[sz] = W, V
-        # ---- batched rank rule on the float32 singular values
-        Sflat = torch.empty(int(s.sum()), dtype=torch.float32, device=self.dev)
+                W, V = heev_small(T, orders=_h2d(s[idx].astype(np.int64), self.dev))
+                ok = finite_rows(W, V)
+            W_all[sz], V_all[sz] = W.contiguous(), V.contiguous()
+            oks.append(ok)
         soff, _ = _prefix(s)
-        Sf64 = {}
-        for sz, (idx, T) in buckets.items():
-            sv = torch.sqrt(torch.clamp(W_all[sz], min=0.0))       # [B, sz] float64
-            Sf64[sz] = sv
-            # scatter the first s_g values of each row into Sflat
-            lens = s[idx]
-            rows = np.repeat(np.arange(len(idx)), lens)
-            cols = np.concatenate([np.arange(L) for L in lens])
-            dst = np.concatenate([soff[g] + np.arange(s[g]) for g in idx])
-            Sflat[torch.from_numpy(dst).to(self.dev)] = sv[torch.from_numpy(rows).to(self.dev),
-                                                           torch.from_numpy(cols).to(self.dev)].float()
-        sp = torch.from_numpy((Sflat.data_ptr() + 4 * soff).astype(np.int64)).to(self.dev)
-        md = torch.from_numpy(s.astype(np.int64)).to(self.dev)
-        cd = torch.from_numpy(self.cap[jj].astype(np.int64)).to(self.dev)
-        res = torch.empty(G * ctypes.sizeof(RankResult), dtype=torch.uint8, device=self.dev)
-        check(lib().aq_rank_select_batched(_ffi.AQ_C64, _vp(sp), _vp(md), _vp(cd), self.eps, G,
-                                           _vp(res), _stream()), "aq_rank_select_batched")
-        rr = np.frombuffer(res.cpu().numpy().tobytes(), dtype=np.dtype(
-            [("k", "i4"), ("flags", "i4"), ("eps_local", "f8"), ("entropy", "f8"),
-             ("condS", "f8"), ("total", "f8"), ("kept", "f8")]))
+        rr, okh = self._rank_all(buckets, W_all, s, soff, jj, torch.cat(oks))
+        if not okh.all():
+            # rare: refill the failed rows from their theta's SVD, rank again
+            at = 0
+            for sz, (idx, T) in buckets.items():
+                bad = np.flatnonzero(~okh[at: at + len(idx)]).tolist()
+                at += len(idx)
+                if bad:
+                    self.eig_fallbacks += len(bad)
+                    self._svd_fill(W_all[sz], V_all[sz], bad, idx, X64, xoff, m, n, left)
+            rr, _ = self._rank_all(buckets, W_all, s, soff, jj, None)
+        Sf64 = self._last_sv
         k = rr["k"].astype(np.int64)
         # ---- projections (FP64) and rebuild into the slots
         vptr = np.zeros(G, np.int64)
@@ -326,8 +368,8 @@ class BatchedMPS:
         # new shapes: site b -> [dl, 2, k], site b+1 -> [k, 2, dr]
         cp = np.zeros(2 * G, COPY_DESC)
         A_, B_ = cp[:G], cp[G:]
-        A_["dst"] = [self._ptr(j, b) for j, b in zip(jj, bb)]
-        B_["dst"] = [self._ptr(j, b + 1) for j, b in zip(jj, bb)]
+        A_["dst"] = self._ptrs(jj, bb)
+        B_["dst"] = self._ptrs(jj, bb + 1)
         # left: A' = V[:, :k] * S (col scale), B' = P * (1/S) (row scale)
         A_["src"] = np.where(left, vptr, P.data_ptr() + 16 * poff)
         A_["rows"], A_["cols"] = m, k
@@ -343,7 +385,7 @@ class BatchedMPS:
         B_["row_scale"] = np.where(left, iptr, 0)
         cnt = cp["rows"] * cp["cols"]
         pre, total = _prefix(cnt)
-        dd, pd = _upload(cp, self.dev), torch.from_numpy(pre).to(self.dev)
+        dd, pd = _upload(cp, self.dev), _h2d(pre, self.dev)
 
         def runc(g):
             check(lib().aq_grouped_copy(_vp(dd), _vp(pd), len(cp), total, g, _stream()),
@@ -356,7 +398,41 @@ class BatchedMPS:
         self.logs["eps_local"].extend(rr["eps_local"].tolist())
         self.logs["entropy"].extend(rr["entropy"].tolist())
         # keep the temporaries alive until the stream has consumed them
-        self._keep = (X, X64, buckets, W_all, V_all, Sflat, Sf64, inv, P, dd, pd)
+        self._keep = (X, X64, buckets, W_all, V_all, self._keep_rank, Sf64, inv, P, dd, pd)
+
+    def _rank_all(self, buckets, W_all, s, soff, jj, ok_dev):
+        """Batched rank rule on the float32 singular values of every bucket;
+        returns the host results and, when ``ok_dev`` is given, the host copy
+        of those per-matrix flags (fetched in the same transfer)."""
+        G = len(jj)
+        Sflat = torch.empty(int(s.sum()), dtype=torch.float32, device=self.dev)
+        Sf64 = {}
+        for sz, (idx, T) in buckets.items():
+            sv = torch.sqrt(torch.clamp(W_all[sz], min=0.0))       # [B, sz] float64
+            Sf64[sz] = sv
+            # scatter the first s_g values of each row into Sflat
+            lens = s[idx]
+            rows = np.repeat(np.arange(len(idx)), lens)
+            cols = _ragged_arange(lens)
+            dst = np.repeat(soff[idx], lens) + cols
+            Sflat[_h2d(dst, self.dev)] = sv[_h2d(rows, self.dev), _h2d(cols, self.dev)].float()
+        sp = _h2d((Sflat.data_ptr() + 4 * soff).astype(np.int64), self.dev)
+        md = _h2d(s.astype(np.int64), self.dev)
+        cd = _h2d(self.cap[jj].astype(np.int64), self.dev)
+        nres = G * ctypes.sizeof(RankResult)
+        nok = 0 if ok_dev is None else int(ok_dev.numel())
+        buf = torch.zeros(nres + nok, dtype=torch.uint8, device=self.dev)
+        check(lib().aq_rank_select_batched(_ffi.AQ_C64, _vp(sp), _vp(md), _vp(cd), self.eps, G,
+                                           _vp(buf), _stream()), "aq_rank_select_batched")
+        if nok:
+            buf[nres:] = ok_dev.to(torch.uint8)
+        host = buf.cpu().numpy()
+        rr = np.frombuffer(host[:nres].tobytes(), dtype=np.dtype(
+            [("k", "i4"), ("flags", "i4"), ("eps_local", "f8"), ("entropy", "f8"),
+             ("condS", "f8"), ("total", "f8"), ("kept", "f8")]))
+        self._last_sv = Sf64
+        self._keep_rank = (Sflat, sp, md, cd, buf)
+        return rr, (host[nres:] != 0 if nok else None)
 
     @staticmethod
     def _svd_fill(W, V, bad, idx, X64, xoff, m, n, left):
@@ -379,35 +455,55 @@ class BatchedMPS:
     def run_hea(self, circuits):
         """circuits[j]: op list of atlas_q.circuits.hea (same layer structure for
         all MPS: 2n 1q ops then CZ on even bonds then odd bonds per layer)."""
-        pos = [0] * self.M
-        while True:
-            any_left = False
-            one = [[] for _ in range(self.M)]
-            for j, ops in enumerate(circuits):
-                # a 1q block: consecutive 1q ops, merged per site (U^T convention:
-                # applying U1 then U2 equals applying U1 @ U2)
-                merged = {}
-                while pos[j] < len(ops) and ops[pos[j]][0] == "1q":
-                    _, q, U = ops[pos[j]]
-                    merged[q] = merged[q] @ U if q in merged else np.asarray(U)
-                    pos[j] += 1
-                one[j] = sorted(merged.items())
-                any_left |= bool(one[j])
-            self.apply_1q_layer(one)
-            two = [[] for _ in range(self.M)]
-            for j, ops in enumerate(circuits):
-                used = set()
-                while pos[j] < len(ops) and ops[pos[j]][0] == "2q":
-                    b = ops[pos[j]][1]
+        self.run_compiled(self.compile_circuits(circuits))
+
+    @staticmethod
+    def compile_circuits(circuits):
+        """Split each op list into steps (a block of consecutive 1q ops, then 2q
+        ops up to the first one whose bond touches an earlier bond of the
+        block), as arrays: per circuit a list of (q [L], U [L, 2, 2], bonds
+        [G], U4 [G, 4, 4])."""
+        out = []
+        for ops in circuits:
+            steps, pos, L = [], 0, len(ops)
+            while pos < L:
+                q1, u1 = [], []
+                while pos < L and ops[pos][0] == "1q":
+                    q1.append(ops[pos][1])
+                    u1.append(ops[pos][2])
+                    pos += 1
+                b2, u2, used = [], [], set()
+                while pos < L and ops[pos][0] == "2q":
+                    b = ops[pos][1]
                     if b in used or b - 1 in used or b + 1 in used:
                         break
                     used.add(b)
-                    two[j].append((b, ops[pos[j]][2]))
-                    pos[j] += 1
-                any_left |= bool(two[j])
-            self.apply_2q_layer(two)
-            if not any_left:
-                break
+                    b2.append(b)
+                    u2.append(ops[pos][2])
+                    pos += 1
+                steps.append((np.asarray(q1, np.int64),
+                              np.asarray(u1, np.complex128).reshape(-1, 2, 2),
+                              np.asarray(b2, np.int64),
+                              np.asarray(u2, np.complex128).reshape(-1, 4, 4)))
+            out.append(steps)
+        return out
+
+    def run_compiled(self, compiled):
+        """Apply ``compile_circuits`` output: per step, one grouped 1q launch
+        (the block's gates merged per site: applying U1 then U2 equals applying
+        U1 @ U2 in the U^T convention) and one two-qubit sub-layer."""
+        nsteps = max((len(c) for c in compiled), default=0)
+        for t in range(nsteps):
+            live = [(j, c[t]) for j, c in enumerate(compiled) if t < len(c)]
+            j1 = np.concatenate([np.full(len(st[0]), j, np.int64) for j, st in live])
+            if len(j1):
+                q1 = np.concatenate([st[0] for _, st in live])
+                u1 = np.concatenate([st[1] for _, st in live])
+                self.apply_1q_arrays(*_merge_per_site(j1, q1, u1, self.nmax))
+            j2 = np.concatenate([np.full(len(st[2]), j, np.int64) for j, st in live])
+            if len(j2):
+                self.apply_2q_arrays(j2, np.concatenate([st[2] for _, st in live]),
+                                     np.concatenate([st[3] for _, st in live]))
 
     # ------------------------------------------------------------ sampling
     def prepare_sampler(self, shots: int):
@@ -450,8 +546,8 @@ class BatchedMPS:
             tiles = _cdiv(np.full(len(js), S), 64) * _cdiv(d["chiR"], 64)
             tpre, ttot = _prefix(tiles)
             spre, stot = _prefix(np.full(len(js), S))
-            self._site_lists.append((_upload(d, self.dev), torch.from_numpy(tpre).to(self.dev),
-                                     torch.from_numpy(spre).to(self.dev), len(js), ttot, stot))
+            self._site_lists.append((_upload(d, self.dev), _h2d(tpre, self.dev),
+                                     _h2d(spre, self.dev), len(js), ttot, stot))
         self._grid_sample = self.tuner.grid("sample", self._site_lists[0][4], None)
 
     def sweep(self):

@@ -33,7 +33,8 @@ def heevd_batched(G: torch.Tensor, inplace: bool = False, return_ok: bool = Fals
     complex128 tensor and is overwritten by the eigenvectors).  Without
     ``return_ok`` a matrix that did not converge raises; with it the call
     returns a third value, a host list of per-matrix convergence flags, and
-    the caller handles the failed ones."""
+    the caller handles the failed ones; ``return_ok="device"`` returns those
+    flags as a device bool tensor instead (no synchronisation)."""
     from . import kernels as K
     from ._ffi import check, lib
     if inplace:
@@ -49,11 +50,13 @@ def heevd_batched(G: torch.Tensor, inplace: bool = False, return_ok: bool = Fals
     check(lib().aq_heevd_batched(n, K._ptr(A), n * n, K._ptr(W), n, K._ptr(E), K._ptr(info),
                                  bsz, K._stream(A)), "aq_heevd_batched")
     good = (info == 0) & finite_rows(W, A)
+    # row j of A holds conj(v_j): V = A^H, columns ascending -> flip to descending
+    lam, V = W.flip(-1), A.transpose(-2, -1).conj().flip(-1)
+    if return_ok == "device":
+        return lam, V, good
     ok = good.cpu().tolist()
     if not return_ok and not all(ok):
         raise RuntimeError(f"aq_heevd_batched: not converged or non-finite (info {info.tolist()})")
-    # row j of A holds conj(v_j): V = A^H, columns ascending -> flip to descending
-    lam, V = W.flip(-1), A.transpose(-2, -1).conj().flip(-1)
     return (lam, V, ok) if return_ok else (lam, V)
 
 

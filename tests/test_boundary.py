@@ -161,3 +161,42 @@ def test_rank_width_censored_at_training_edge():
         assert p.predict_width(torch.zeros(32), cap, 4096) == want, (y, cap)
     d = np.load(rank_mlp.DEFAULT_WEIGHTS, allow_pickle=False)
     assert "log2_max" not in d.files or float(d["log2_max"]) >= 10.0
+
+
+def test_batched_circuit_compile_and_site_merge():
+    """The batched engine's host steps: circuits split into (1q block, 2q
+    block on disjoint bonds) steps as a plain op-by-op walk does, and a
+    block's 1q gates merged per site in op order (U1 then U2 -> U1 @ U2)."""
+    from atlas_q.batched import BatchedMPS, _merge_per_site
+    circs = [C.hea(n, 3, 40 + n) for n in (4, 7, 12)]
+    comp = BatchedMPS.compile_circuits(circs)
+    for ops, steps in zip(circs, comp):
+        pos, t = 0, 0
+        while pos < len(ops):
+            q1 = []
+            while pos < len(ops) and ops[pos][0] == "1q":
+                q1.append((ops[pos][1], ops[pos][2]))
+                pos += 1
+            b2, used = [], set()
+            while pos < len(ops) and ops[pos][0] == "2q":
+                b = ops[pos][1]
+                if used & {b - 1, b, b + 1}:
+                    break
+                used.add(b)
+                b2.append((b, ops[pos][2]))
+                pos += 1
+            sq, su, sb, su4 = steps[t]
+            assert sq.tolist() == [q for q, _ in q1] and sb.tolist() == [b for b, _ in b2]
+            assert np.allclose(su, np.array([u for _, u in q1]).reshape(-1, 2, 2))
+            assert np.allclose(su4, np.array([u for _, u in b2]).reshape(-1, 4, 4))
+            t += 1
+        assert t == len(steps)
+    rng = np.random.default_rng(3)
+    jj = np.array([0, 1, 0, 0, 1, 2])
+    qq = np.array([3, 0, 3, 1, 0, 5])
+    U = rng.standard_normal((6, 2, 2)) + 1j * rng.standard_normal((6, 2, 2))
+    mj, mq, M = _merge_per_site(jj, qq, U, nmax=8)
+    got = {(int(a), int(b)): m for a, b, m in zip(mj, mq, M)}
+    assert set(got) == {(0, 3), (1, 0), (0, 1), (2, 5)}
+    assert np.allclose(got[(0, 3)], U[0] @ U[2]) and np.allclose(got[(1, 0)], U[1] @ U[4])
+    assert np.allclose(got[(0, 1)], U[3]) and np.allclose(got[(2, 5)], U[5])
