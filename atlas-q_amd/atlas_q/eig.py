@@ -13,15 +13,17 @@ eigenvalues at zero, as in the Gram matrix of a rank-deficient theta), and
 on such batches it has also returned non-finite eigenvectors with a zero
 info code; both count as a failure here.  As the reference's SVD cascade
 retries with a 1e-12 jitter of the input (linalg_robust.py:31-39),
-``eigh_desc`` retries once on G + E, E a random Hermitian matrix of norm
-~1e-13 ||G||_max, which separates the cluster and moves every eigenvalue by
-at most ||E||_2; ``heevd_batched`` reports the failed matrices to its caller.
+``eigh_desc`` retries on G + E, E a random Hermitian matrix of norm ~1e-13,
+then ~1e-11, times ||G||_max, which separates the cluster and moves every
+eigenvalue by at most ||E||_2; ``heevd_batched`` reports the failed matrices
+to its caller.
 """
 from __future__ import annotations
 
 import torch
 
 JACOBI_MAX = 64
+JITTERS = (1e-13, 1e-11)   # relative sizes of the retries' Hermitian perturbation
 
 
 def heevd_batched(G: torch.Tensor, inplace: bool = False, return_ok: bool = False):
@@ -83,9 +85,18 @@ def eigh_desc(G: torch.Tensor):
         return heev_small(G)
     try:
         lam, V = _eigh_checked(G)
-    except RuntimeError:
-        scale = 1e-13 * float(torch.diagonal(G, dim1=-2, dim2=-1).real.abs().max())
-        E = torch.randn(G.shape, dtype=G.dtype, device=G.device)
-        E = (E + E.transpose(-2, -1).conj()) * (0.5 * scale / n ** 0.5)
-        lam, V = _eigh_checked(G + E)
-    return lam.flip(-1), V.flip(-1)
+        return lam.flip(-1), V.flip(-1)
+    except RuntimeError as e:
+        err = e
+    dmax = float(torch.diagonal(G, dim1=-2, dim2=-1).real.abs().max())
+    R = torch.randn(G.shape, dtype=G.dtype, device=G.device)
+    R = (R + R.transpose(-2, -1).conj()) * (0.5 / n ** 0.5)
+    # 1e-13, then 1e-11 of max|G_ii|: a large zero cluster (a rank-deficient
+    # projected core) can need the second to come out finite
+    for rel in JITTERS:
+        try:
+            lam, V = _eigh_checked(G + (rel * dmax) * R)
+            return lam.flip(-1), V.flip(-1)
+        except RuntimeError as e:
+            err = e
+    raise err

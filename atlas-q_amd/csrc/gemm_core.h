@@ -13,6 +13,12 @@
 //
 // Complex products use the 4-multiplication form on real MFMA:
 //   Cr += Ar Br - Ai Bi,  Ci += Ar Bi + Ai Br
+// or, where a kernel has the registers for a third accumulator set
+// (ThreeM = true), the 3-multiplication form:
+//   P1 += Ar Br,  P2 += Ai Bi,  P3 += (Ar + Ai)(Br + Bi),
+//   Cr = P1 - P2,  Ci = P3 - P1 - P2   (after the k loop)
+// -- a quarter fewer MFMAs, absolute error of the same order as the
+// 4-multiplication form (a few units of the FP32 rounding of |A| |B|).
 // FP32: v_mfma_f32_32x32x2f32 (one 32x32 block per wave and half);
 // FP64: v_mfma_f64_16x16x4f64 (2 x 2 blocks of 16x16 per wave and half).
 // Operand tiles are staged through LDS as separate real / imaginary planes,
@@ -162,6 +168,7 @@ template <typename T> struct Frag {
   static constexpr int JB = 32 / E;
   typename Mfma<T>::acc_t re[RB][JB][2];
   typename Mfma<T>::acc_t im[RB][JB][2];
+  typename Mfma<T>::acc_t p3[RB][JB][2];   // 3M form only
 };
 
 template <typename T> struct alignas(16) Smem {
@@ -173,7 +180,7 @@ template <typename T> struct alignas(16) Smem {
 
 // k_fast_a: the A functor is contiguous along k (else along m).
 // n_fast_b: the B functor is contiguous along n (else along k).
-template <typename T, bool KFastA, bool NFastB, class AOp, class BOp>
+template <typename T, bool KFastA, bool NFastB, bool ThreeM = false, class AOp, class BOp>
 __device__ __forceinline__ void cgemm_mainloop(const AOp& aop, const BOp& bop, const ColMap& cm,
                                                int64_t K, int64_t m0, int64_t j0, Smem<T>& sm,
                                                Frag<T>& fr) {
@@ -197,6 +204,7 @@ __device__ __forceinline__ void cgemm_mainloop(const AOp& aop, const BOp& bop, c
         for (int r = 0; r < M::NREG; ++r) {
           fr.re[rb][jb][h][r] = 0;
           fr.im[rb][jb][h][r] = 0;
+          if (ThreeM) fr.p3[rb][jb][h][r] = 0;
         }
 
   // global column of each B element this thread loads is loop-invariant
@@ -253,13 +261,14 @@ __device__ __forceinline__ void cgemm_mainloop(const AOp& aop, const BOp& bop, c
 #pragma unroll
     for (int kk = 0; kk < BK; kk += M::TK) {
       const int kr = kk + M::op_k(lane);
-      T ar[Frag<T>::RB], ai[Frag<T>::RB], nai[Frag<T>::RB];
+      T ar[Frag<T>::RB], ai[Frag<T>::RB], nai[Frag<T>::RB], as[Frag<T>::RB];
 #pragma unroll
       for (int rb = 0; rb < Frag<T>::RB; ++rb) {
         const int row = 32 * wm + rb * E + M::a_row(lane);
         ar[rb] = sm.a[0][kr][row];
         ai[rb] = sm.a[1][kr][row];
         nai[rb] = -ai[rb];
+        as[rb] = ar[rb] + ai[rb];
       }
 #pragma unroll
       for (int h = 0; h < 2; ++h)
@@ -268,15 +277,37 @@ __device__ __forceinline__ void cgemm_mainloop(const AOp& aop, const BOp& bop, c
           const int col = h * kBJ + 32 * wn + jb * E + M::op_idx(lane);
           const T br = sm.b[0][kr][col];
           const T bi = sm.b[1][kr][col];
+          const T bs = br + bi;
 #pragma unroll
           for (int rb = 0; rb < Frag<T>::RB; ++rb) {
-            fr.re[rb][jb][h] = M::mma(ar[rb], br, fr.re[rb][jb][h]);
-            fr.re[rb][jb][h] = M::mma(nai[rb], bi, fr.re[rb][jb][h]);
-            fr.im[rb][jb][h] = M::mma(ar[rb], bi, fr.im[rb][jb][h]);
-            fr.im[rb][jb][h] = M::mma(ai[rb], br, fr.im[rb][jb][h]);
+            if (ThreeM) {
+              fr.re[rb][jb][h] = M::mma(ar[rb], br, fr.re[rb][jb][h]);   // P1
+              fr.im[rb][jb][h] = M::mma(ai[rb], bi, fr.im[rb][jb][h]);   // P2
+              fr.p3[rb][jb][h] = M::mma(as[rb], bs, fr.p3[rb][jb][h]);   // P3
+            } else {
+              fr.re[rb][jb][h] = M::mma(ar[rb], br, fr.re[rb][jb][h]);
+              fr.re[rb][jb][h] = M::mma(nai[rb], bi, fr.re[rb][jb][h]);
+              fr.im[rb][jb][h] = M::mma(ar[rb], bi, fr.im[rb][jb][h]);
+              fr.im[rb][jb][h] = M::mma(ai[rb], br, fr.im[rb][jb][h]);
+            }
           }
         }
     }
+  }
+  if (ThreeM) {
+    // Cr = P1 - P2, Ci = P3 - P1 - P2
+#pragma unroll
+    for (int rb = 0; rb < Frag<T>::RB; ++rb)
+#pragma unroll
+      for (int jb = 0; jb < Frag<T>::JB; ++jb)
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+          for (int r = 0; r < M::NREG; ++r) {
+            const T p1 = fr.re[rb][jb][h][r], p2 = fr.im[rb][jb][h][r];
+            fr.re[rb][jb][h][r] = p1 - p2;
+            fr.im[rb][jb][h][r] = fr.p3[rb][jb][h][r] - p1 - p2;
+          }
   }
 }
 

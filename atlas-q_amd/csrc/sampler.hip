@@ -31,7 +31,8 @@ __global__ __launch_bounds__(kThreads, 2) void sample_gemm_kernel(
   BStrided<T> bop{site, 2 * chiR, 1, chiL, 2 * chiR, 0};
   ColMap cm{1, chiR};
   Frag<T> fr;
-  cgemm_mainloop<T, true, true>(aop, bop, cm, chiL, m0, j0, sm, fr);
+  cgemm_mainloop<T, true, true, std::is_same<T, float>::value>(aop, bop, cm, chiL, m0, j0, sm,
+                                                               fr);
   sampler_epilogue<T>(fr, Tout, part, S, chiR, m0, j0, ntiles);
 }
 

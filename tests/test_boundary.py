@@ -200,3 +200,30 @@ def test_batched_circuit_compile_and_site_merge():
     assert set(got) == {(0, 3), (1, 0), (0, 1), (2, 5)}
     assert np.allclose(got[(0, 3)], U[0] @ U[2]) and np.allclose(got[(1, 0)], U[1] @ U[4])
     assert np.allclose(got[(0, 1)], U[3]) and np.allclose(got[(2, 5)], U[5])
+
+
+def test_eigh_retry_ladder(monkeypatch):
+    """eigh_desc retries on G + E with E ~ 1e-13 then 1e-11 of max|G_ii|
+    (rank-deficient cores: a large zero cluster); the third attempt's
+    eigenvalues stay within 1e-10 of the unperturbed ones."""
+    from atlas_q import eig
+    rng = np.random.default_rng(8)
+    X = rng.standard_normal((80, 30)) + 1j * rng.standard_normal((80, 30))
+    G = torch.from_numpy(X @ X.conj().T)                  # rank 30 of 80
+    real = eig._eigh_checked
+    calls = {"n": 0}
+
+    def flaky(A):
+        calls["n"] += 1
+        if calls["n"] < 3:
+            raise RuntimeError("linalg.eigh: non-finite eigenpairs")
+        return real(A)
+    monkeypatch.setattr(eig, "_eigh_checked", flaky)
+    lam, V = eig.eigh_desc(G)
+    assert calls["n"] == 3
+    ref = np.linalg.eigvalsh(G.numpy())[::-1]
+    assert np.abs(lam.numpy() - ref).max() <= 1e-10 * np.abs(ref).max()
+    calls["n"] = -10                                       # every attempt fails
+    monkeypatch.setattr(eig, "_eigh_checked", lambda A: (_ for _ in ()).throw(RuntimeError("x")))
+    with pytest.raises(RuntimeError):
+        eig.eigh_desc(G)
