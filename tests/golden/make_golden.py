@@ -66,11 +66,12 @@ def stats_arrays(mps):
     )
 
 
-def run_brickwork(name, n, depth, seed, dtype, chi, eps, budget=None, n_amp=None):
+def run_brickwork(name, n, depth, seed, dtype, chi, eps, budget=None, n_amp=None, promote=None):
     torch.manual_seed(1234)
+    policy = DTypePolicy(default=dtype) if promote is None else \
+        DTypePolicy(default=dtype, promote_if_cond_gt=promote)
     mps = AdaptiveMPS(n, bond_dim=1, eps_bond=eps, chi_max_per_bond=chi,
-                      budget_global_mb=budget,
-                      dtype_policy=DTypePolicy(default=dtype), device="cpu")
+                      budget_global_mb=budget, dtype_policy=policy, device="cpu")
     ops = brickwork(n, depth, seed)
     for b, U in ops:
         mps.apply_two_site_gate(b, torch.from_numpy(U).to(dtype))
@@ -84,7 +85,8 @@ def run_brickwork(name, n, depth, seed, dtype, chi, eps, budget=None, n_amp=None
         out.update(psi=psi)
     out["bond_dims"] = np.array(mps.bond_dims, np.int64)
     out.update(n=n, depth=depth, seed=seed, chi=chi, eps=eps,
-               dtype=str(dtype), budget=-1.0 if budget is None else budget)
+               dtype=str(dtype), budget=-1.0 if budget is None else budget,
+               promote=-1.0 if promote is None else promote)
     np.savez_compressed(HERE / f"{name}.npz", **out)
     print(name, "updates", len(out["k"]), "max k", out["k"].max(), flush=True)
     return mps
@@ -161,7 +163,18 @@ def run_sampler(name, n, depth, seed, dtype, chi, eps, shots):
     print(name, "shots", shots, flush=True)
 
 
-if __name__ == "__main__":
+RUNS = {
+    # precision promotion (adaptive_mps.py:268-274): with the default 1e6 a
+    # complex64 run never promotes (the float32 rule keeps cond(S) < ~4e3)
+    "brick_n8_promote": lambda: run_brickwork("brick_n8_promote", 8, 8, 16, torch.complex64,
+                                              16, 1e-6, promote=10.0),
+}
+
+if __name__ == "__main__" and len(sys.argv) > 1:
+    # only the named runs (the other fixtures are left as they are)
+    for name in sys.argv[1:]:
+        RUNS[name]()
+elif __name__ == "__main__":
     run_brickwork("brick_n8_c128", 8, 8, 11, torch.complex128, 4, 1e-10)
     run_brickwork("brick_n10_c64", 10, 10, 12, torch.complex64, 8, 1e-6)
     run_brickwork("brick_n10_c64_eps", 10, 6, 13, torch.complex64, 32, 3e-2)
@@ -173,5 +186,7 @@ if __name__ == "__main__":
     run_init("init_n5_bd3_c128", 5, 3, torch.complex128, 6)
     run_sampler("sampler_n12_c64", 12, 8, 31, torch.complex64, 16, 1e-6, 512)
     run_sampler("sampler_n12_c128", 12, 8, 32, torch.complex128, 16, 1e-10, 256)
+    for run in RUNS.values():
+        run()
     (HERE / "meta.json").write_text(json.dumps(META, indent=1) + "\n")
     print(META)

@@ -25,11 +25,12 @@ MARGIN = {torch.complex64: 1e-5, torch.complex128: 1e-10}
 DTYPES = {"torch.complex64": torch.complex64, "torch.complex128": torch.complex128}
 
 
-def _run_brickwork(g, dtype, budget=None, **kw):
+def _run_brickwork(g, dtype, budget=None, promote=None, **kw):
     n = int(g["n"])
+    policy = DTypePolicy(default=dtype) if promote is None else \
+        DTypePolicy(default=dtype, promote_if_cond_gt=promote)
     m = AdaptiveMPS(n, bond_dim=1, eps_bond=float(g["eps"]), chi_max_per_bond=int(g["chi"]),
-                    budget_global_mb=budget, dtype_policy=DTypePolicy(default=dtype),
-                    device=DEV, **kw)
+                    budget_global_mb=budget, dtype_policy=policy, device=DEV, **kw)
     for b, U in brickwork(n, int(g["depth"]), int(g["seed"])):
         m.apply_two_site_gate(b, torch.from_numpy(U).to(dtype))
     return m
@@ -282,3 +283,23 @@ def test_runner_ghz_counts_small_and_beyond_64_qubits():
             assert sv is None
     counts, _ = run_mps([("x", [0], []), ("ry", [2], [np.pi])], 3, 50, seed=1, born_exact=True)
     assert counts == {"101": 50}
+
+
+@pytest.mark.parametrize("method", ["plain", "gram"])
+def test_precision_promotion_matches_reference_golden(method):
+    """adaptive_mps.py:268-274 with promote_if_cond_gt = 10 on a complex64
+    brickwork (golden run written by the reference): the updates whose
+    cond(S) exceeds the threshold are redone in complex128 and committed as
+    complex128 cores -- the same updates as the reference's, the same ranks
+    and amplitudes to the complex64 tolerance.  The exact Gram method
+    promotes through the same policy."""
+    g = load_golden("brick_n8_promote")
+    m = _run_brickwork(g, torch.complex64, None, svd_method=method,
+                       promote=float(g["promote"]))
+    promoted = [d == "torch.complex128" for d in m.statistics.logs["dtype"]]
+    assert promoted == g["promoted"].tolist() and any(promoted)
+    assert m.statistics.logs["k_star"] == g["k"].tolist()
+    np.testing.assert_allclose(m.statistics.logs["condS"], g["condS"], rtol=1e-3)
+    psi = m.to_statevector().cpu().numpy().astype(np.complex128)
+    err = np.abs(psi - g["psi"]).max() / np.abs(g["psi"]).max()
+    assert err < AMP_TOL[torch.complex64] * 10, err

@@ -91,8 +91,11 @@ def test_theta_closed_form(dtype):
 def _replay_brickwork(g, dtype, budget=None):
     torch.manual_seed(1234)
     n = int(g["n"])
+    promote = float(g["promote"]) if "promote" in g and float(g["promote"]) > 0 else None
+    policy = R.DTypePolicy(default=dtype) if promote is None else \
+        R.DTypePolicy(default=dtype, promote_if_cond_gt=promote)
     m = R.OracleMPS(n, bond_dim=1, eps_bond=float(g["eps"]), chi_max_per_bond=int(g["chi"]),
-                    budget_global_mb=budget, dtype_policy=R.DTypePolicy(default=dtype))
+                    budget_global_mb=budget, dtype_policy=policy)
     for b, U in brickwork(n, int(g["depth"]), int(g["seed"])):
         m.apply_two_site_gate(b, torch.from_numpy(U).to(dtype))
     return m
@@ -105,6 +108,7 @@ def _replay_brickwork(g, dtype, budget=None):
     ("brick_n10_c128_eps", torch.complex128, None),
     ("brick_n8_budget", torch.complex64, 0.0025),
     ("config1_n16_c128", torch.complex128, None),
+    ("brick_n8_promote", torch.complex64, None),
 ])
 def test_oracle_matches_reference_golden(name, dtype, budget):
     g = load_golden(name)
