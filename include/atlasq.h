@@ -269,6 +269,9 @@ typedef struct {
 int aq_sample_grouped_site(const aq_sample_desc_t* descs, const int64_t* tile_prefix,
                            const int64_t* shot_prefix, int count, int64_t total_tiles,
                            int64_t total_shots, int grid, hipStream_t stream);
+/* Zero `count` uint64 words on `stream` (no reference counterpart): the
+ * sampler's bit words, and a one-element launch that brackets a timed region
+ * in a kernel trace. */
 int aq_zero_u64(uint64_t* p, int64_t count, hipStream_t stream);
 
 /* Grouped core rebuild (adaptive_mps.py:280-290 for many updates): complex64
