@@ -159,7 +159,45 @@ __global__ __launch_bounds__(NT) void heev_jacobi_kernel(int n, const int64_t* _
         sm.sp[tid] = cscale(e, s);
       }
       __syncthreads();
-      // columns: A <- A J, V <- V J   (item = row r, pair k)
+      // A <- J^H A J by 2 x 2 blocks: the rotations of a round act on disjoint
+      // pairs, so block (pair kr, pair kc) -- rows {p_r, q_r} x columns
+      // {p_c, q_c} -- maps to J_r^H A[block] J_c from its own old values
+      // (one pass over A, no intermediate barrier)
+      for (int it = tid; it < half * half; it += blockDim.x) {
+        const int kr = it / half, kc = it % half;
+        const int pr = sm.pp[kr], qr = sm.pq[kr], pc = sm.pp[kc], qc = sm.pq[kc];
+        const double cc = sm.cs[kc];
+        const double sc = sm.sp[kc].x * sm.ep[kc].x + sm.sp[kc].y * sm.ep[kc].y;
+        const double2 ec = sm.ep[kc], sec = sm.sp[kc];
+        const double cr = sm.cs[kr];
+        const double sr = sm.sp[kr].x * sm.ep[kr].x + sm.sp[kr].y * sm.ep[kr].y;
+        const double2 erc = cconj(sm.ep[kr]), serc = cconj(sm.sp[kr]);
+        const double2 a00 = sm.a[pr][pc], a01 = sm.a[pr][qc];
+        const double2 a10 = sm.a[qr][pc], a11 = sm.a[qr][qc];
+        // columns (A J_c): new p = c ap - s e aq ; new q = s ap + c e aq
+        const double2 b00 = cadd2(cscale(a00, cc), cscale(cm(sec, a01), -1.0));
+        const double2 b01 = cadd2(cscale(a00, sc), cscale(cm(ec, a01), cc));
+        const double2 b10 = cadd2(cscale(a10, cc), cscale(cm(sec, a11), -1.0));
+        const double2 b11 = cadd2(cscale(a10, sc), cscale(cm(ec, a11), cc));
+        // rows (J_r^H .): new p = c bp - s conj(e) bq ; new q = s bp + c conj(e) bq
+        double2 n00 = cadd2(cscale(b00, cr), cscale(cm(serc, b10), -1.0));
+        double2 n10 = cadd2(cscale(b00, sr), cscale(cm(erc, b10), cr));
+        double2 n01 = cadd2(cscale(b01, cr), cscale(cm(serc, b11), -1.0));
+        double2 n11 = cadd2(cscale(b01, sr), cscale(cm(erc, b11), cr));
+        if (kr == kc) {
+          // the rotated pair is exactly diagonalised: clear the rounding in
+          // its off-diagonal entries and the imaginary parts of its diagonal
+          n01 = make_double2(0.0, 0.0);
+          n10 = make_double2(0.0, 0.0);
+          n00.y = 0.0;
+          n11.y = 0.0;
+        }
+        sm.a[pr][pc] = n00;
+        sm.a[pr][qc] = n01;
+        sm.a[qr][pc] = n10;
+        sm.a[qr][qc] = n11;
+      }
+      // V <- V J   (item = row r, pair k; V is not read by the A pass)
       for (int it = tid; it < ne * half; it += blockDim.x) {
         const int k = it / ne, r = it % ne;
         const int p = sm.pp[k], q = sm.pq[k];
@@ -167,38 +205,9 @@ __global__ __launch_bounds__(NT) void heev_jacobi_kernel(int n, const int64_t* _
         const double s = sm.sp[k].x * sm.ep[k].x + sm.sp[k].y * sm.ep[k].y;  // |s|, sign kept
         const double2 e = sm.ep[k];
         const double2 se = sm.sp[k];
-        const double2 ap = sm.a[r][p], aq = sm.a[r][q];
-        // new p = c ap - s e aq ; new q = s ap + c e aq
-        sm.a[r][p] = cadd2(cscale(ap, c), cscale(cm(se, aq), -1.0));
-        sm.a[r][q] = cadd2(cscale(ap, s), cscale(cm(e, aq), c));
         const double2 vp = sm.v[r][p], vq = sm.v[r][q];
         sm.v[r][p] = cadd2(cscale(vp, c), cscale(cm(se, vq), -1.0));
         sm.v[r][q] = cadd2(cscale(vp, s), cscale(cm(e, vq), c));
-      }
-      __syncthreads();
-      // rows: A <- J^H A   (item = pair k, column r)
-      for (int it = tid; it < ne * half; it += blockDim.x) {
-        const int k = it / ne, r = it % ne;
-        const int p = sm.pp[k], q = sm.pq[k];
-        const double c = sm.cs[k];
-        const double s = sm.sp[k].x * sm.ep[k].x + sm.sp[k].y * sm.ep[k].y;
-        const double2 ec = cconj(sm.ep[k]);
-        const double2 sec = cconj(sm.sp[k]);
-        const double2 ap = sm.a[p][r], aq = sm.a[q][r];
-        // new p = c ap - s conj(e) aq ; new q = s ap + c conj(e) aq
-        double2 np_ = cadd2(cscale(ap, c), cscale(cm(sec, aq), -1.0));
-        double2 nq_ = cadd2(cscale(ap, s), cscale(cm(ec, aq), c));
-        // the rotated pair is exactly diagonalised: clear the rounding in
-        // its off-diagonal entries and the imaginary parts of its diagonal
-        if (r == p) {
-          np_.y = 0.0;
-          nq_ = make_double2(0.0, 0.0);
-        } else if (r == q) {
-          np_ = make_double2(0.0, 0.0);
-          nq_.y = 0.0;
-        }
-        sm.a[p][r] = np_;
-        sm.a[q][r] = nq_;
       }
       __syncthreads();
     }
