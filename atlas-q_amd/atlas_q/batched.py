@@ -71,6 +71,23 @@ def _cdiv(a, b):
     return -(-a // b)
 
 
+def _gather_gates(us, d: int) -> np.ndarray:
+    """Stack d x d gate matrices into a complex128 array [len(us), d, d]: their
+    bytes joined when the gates are complex128 arrays of d * d entries (the
+    common case, ~3x faster than numpy.array on the list; checked on the first
+    gate's dtype and on the total byte count), else converted."""
+    if not us:
+        return np.zeros((0, d, d), np.complex128)
+    if getattr(us[0], "dtype", None) == np.complex128:
+        try:
+            buf = b"".join([u.tobytes() for u in us])
+        except AttributeError:
+            buf = b""
+        if len(buf) == 16 * d * d * len(us):
+            return np.frombuffer(buf, np.complex128).reshape(len(us), d, d).copy()
+    return np.array(us, np.complex128).reshape(len(us), d, d)
+
+
 def _merge_per_site(jj: np.ndarray, qq: np.ndarray, U: np.ndarray, nmax: int):
     """Gates U[g] on sites (jj[g], qq[g]) in op order, merged per site into one
     matrix per site: U_first @ ... @ U_last.  Returns (jj, qq, U) per site."""
@@ -462,29 +479,51 @@ class BatchedMPS:
         """Split each op list into steps (a block of consecutive 1q ops, then 2q
         ops up to the first one whose bond touches an earlier bond of the
         block), as arrays: per circuit a list of (q [L], U [L, 2, 2], bonds
-        [G], U4 [G, 4, 4])."""
+        [G], U4 [G, 4, 4]).  Runs of 1q ops are sliced from per-circuit
+        arrays; only the 2q runs are walked, for the bond-conflict rule."""
         out = []
+        z1 = (np.zeros(0, np.int64), np.zeros((0, 2, 2), np.complex128))
+        z2 = (np.zeros(0, np.int64), np.zeros((0, 4, 4), np.complex128))
         for ops in circuits:
-            steps, pos, L = [], 0, len(ops)
-            while pos < L:
-                q1, u1 = [], []
-                while pos < L and ops[pos][0] == "1q":
-                    q1.append(ops[pos][1])
-                    u1.append(ops[pos][2])
-                    pos += 1
-                b2, u2, used = [], [], set()
-                while pos < L and ops[pos][0] == "2q":
-                    b = ops[pos][1]
-                    if b in used or b - 1 in used or b + 1 in used:
-                        break
-                    used.add(b)
-                    b2.append(b)
-                    u2.append(ops[pos][2])
-                    pos += 1
-                steps.append((np.asarray(q1, np.int64),
-                              np.asarray(u1, np.complex128).reshape(-1, 2, 2),
-                              np.asarray(b2, np.int64),
-                              np.asarray(u2, np.complex128).reshape(-1, 4, 4)))
+            L = len(ops)
+            if L == 0:
+                out.append([])
+                continue
+            kinds, wh, us = zip(*ops)
+            two = np.fromiter(map("2q".__eq__, kinds), bool, L)
+            where = np.array(wh, np.int64)
+            i1 = np.flatnonzero(~two)
+            i2 = np.flatnonzero(two)
+            U1 = _gather_gates([us[i] for i in i1], 2)
+            U2 = _gather_gates([us[i] for i in i2], 4)
+            r1 = np.full(L, -1, np.int64)      # op index -> row in U1 / U2
+            r1[i1] = np.arange(len(i1))
+            r2 = np.full(L, -1, np.int64)
+            r2[i2] = np.arange(len(i2))
+            # runs of equal kind: [start, end)
+            cut = np.flatnonzero(np.diff(two.astype(np.int8))) + 1
+            starts = np.concatenate([[0], cut])
+            ends = np.concatenate([cut, [L]])
+            steps, pending = [], None          # pending: the 1q block of the next step
+            for a, b in zip(starts.tolist(), ends.tolist()):
+                if not two[a]:
+                    if pending is not None:    # two 1q runs cannot be adjacent
+                        steps.append(pending + z2)
+                    pending = (where[a:b], U1[r1[a]:r1[a] + (b - a)])
+                    continue
+                bonds = where[a:b].tolist()
+                blk, used = a, set()
+                for t, bnd in enumerate(bonds):
+                    if bnd in used or bnd - 1 in used or bnd + 1 in used:
+                        q1 = pending if pending is not None else z1
+                        steps.append(q1 + (where[blk:a + t], U2[r2[blk]:r2[a + t - 1] + 1]))
+                        pending, blk, used = None, a + t, set()
+                    used.add(bnd)
+                q1 = pending if pending is not None else z1
+                steps.append(q1 + (where[blk:b], U2[r2[blk]:r2[b - 1] + 1]))
+                pending = None
+            if pending is not None:
+                steps.append(pending + z2)
             out.append(steps)
         return out
 

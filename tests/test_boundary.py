@@ -169,6 +169,17 @@ def test_batched_circuit_compile_and_site_merge():
     block's 1q gates merged per site in op order (U1 then U2 -> U1 @ U2)."""
     from atlas_q.batched import BatchedMPS, _merge_per_site
     circs = [C.hea(n, 3, 40 + n) for n in (4, 7, 12)]
+    rng = np.random.default_rng(11)
+    for _ in range(6):                  # random op lists: any mix, conflicts, ragged ends
+        ops = []
+        for _ in range(int(rng.integers(0, 40))):
+            if rng.random() < 0.5:
+                ops.append(("1q", int(rng.integers(0, 8)),
+                            rng.standard_normal((2, 2)) + 1j * rng.standard_normal((2, 2))))
+            else:
+                ops.append(("2q", int(rng.integers(0, 7)),
+                            rng.standard_normal((4, 4)) + 1j * rng.standard_normal((4, 4))))
+        circs.append(ops)
     comp = BatchedMPS.compile_circuits(circs)
     for ops, steps in zip(circs, comp):
         pos, t = 0, 0
