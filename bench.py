@@ -75,7 +75,7 @@ def parse():
     ap.add_argument("--seq-steps", type=int, default=4,
                     help="updates timed one at a time (workers=1) for both paths")
     ap.add_argument("--fidelity-samples", type=int, default=2)
-    ap.add_argument("--cpu-updates", type=int, default=2)
+    ap.add_argument("--cpu-updates", type=int, default=8)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-low", action="store_true",
@@ -98,6 +98,9 @@ def parse():
     ap.add_argument("--wide-sites", type=int, default=12)
     ap.add_argument("--wide-plain-steps", type=int, default=2)
     ap.add_argument("--wide-fidelity", type=int, default=1)
+    ap.add_argument("--wide-cpu-updates", type=int, default=4)
+    ap.add_argument("--batched-cpu-mps", type=int, default=2,
+                    help="MPS of the batched population run by the oracle on the host cores")
     ap.add_argument("--no-wide", action="store_true")
     ap.add_argument("--wide-only", action="store_true")
     ap.add_argument("--batched-only", action="store_true")
@@ -331,6 +334,9 @@ def wide_bench(torch, a, dev):
                                                a.workers)
     out["plain_updates_per_s"] = plain
     out["speedup_vs_plain"] = out["updates_per_s"] / plain if plain else None
+    if not a.no_cpu and a.wide_cpu_updates > 0:
+        out["cpu_baseline"] = cpu_slice_baseline(torch, base, sched, chi, a.wide_cpu_updates,
+                                                 f"{2 * chi}x{2 * chi} complex64 theta")
     fid = []
     if a.wide_fidelity > 0:
         # the fast path as timed (a whole sub-layer, grouped); plain gate by gate
@@ -349,6 +355,43 @@ def wide_bench(torch, a, dev):
     out["max_abs_dF_fast_vs_plain"] = max((abs(f["fast"]["F"] - f["plain"]["F"]) for f in fid),
                                          default=None)
     return out
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_slice_baseline(torch, base, sched, chi, updates, what):
+    """The oracle (torch-CPU restatement of the reference update, LAPACK SVD)
+    on the host cores: one warm-up update, then ``updates`` timed one by one on
+    the slice ``base`` with the gates ``sched``."""
+    import numpy as np
+    from oracle import ref_mps as R
+    cores = len(os.sched_getaffinity(0))
+    torch.set_num_threads(cores)
+    touched = 2 + max(b for b, _ in sched[: 1 + updates])
+    om = R.OracleMPS(touched, bond_dim=1, eps_bond=1e-8, chi_max_per_bond=chi)
+    om.tensors = [t.cpu() for t in base[:touched]]
+    om.bond_dims = [chi] * (touched - 1)
+    b, U = sched[0]
+    om.apply_two_site_gate(b, torch.from_numpy(U).to(torch.complex64))      # warm-up
+    times = []
+    for b, U in sched[1: 1 + updates]:
+        t0 = time.perf_counter()
+        om.apply_two_site_gate(b, torch.from_numpy(U).to(torch.complex64))
+        times.append(time.perf_counter() - t0)
+    return {"value": updates / sum(times), "unit": "updates/s", "cores": cores,
+            "cpu_model": cpu_model(), "kind": "port",
+            "median_update_s": float(np.median(times)),
+            "sample": f"{updates} two-site updates ({what}, LAPACK SVD, rank rule, rebuild) on "
+                      f"the same chi={chi} slice, torch CPU with {cores} threads, after 1 "
+                      f"warm-up update"}
 
 
 def mps_fidelity(torch, ta, tb) -> float:
@@ -448,6 +491,32 @@ def batched_bench(torch, a, dev):
                        "subspaces; the reference run in complex64 and in complex128 differs "
                        "there too, so the truncated states are compared by their global error "
                        "bound sqrt(sum eps_local^2)"}
+    cpu = None
+    if not a.no_cpu and a.batched_cpu_mps > 0:
+        from oracle import ref_mps as R
+        cores = len(os.sched_getaffinity(0))
+        torch.set_num_threads(cores)
+        c_up, c_t = 0, 0.0
+        for j in range(min(a.batched_cpu_mps, len(specs))):
+            n, chi = specs[j]
+            om = R.OracleMPS(n, bond_dim=1, eps_bond=1e-8, chi_max_per_bond=chi)
+            om.tensors = [torch.tensor([1.0, 0.0], dtype=torch.complex64).reshape(1, 2, 1)
+                          for _ in range(n)]
+            om.bond_dims = [1] * (n - 1)
+            t0 = time.perf_counter()
+            for op in circs[j]:
+                U = torch.from_numpy(op[2]).to(torch.complex64)
+                if op[0] == "1q":
+                    om.apply_single_qubit_gate(op[1], U)
+                else:
+                    om.apply_two_site_gate(op[1], U)
+            c_t += time.perf_counter() - t0
+            c_up += sum(1 for op in circs[j] if op[0] == "2q")
+        cpu = {"value": c_up / c_t, "unit": "updates/s", "cores": cores, "cpu_model": cpu_model(),
+               "kind": "port",
+               "sample": f"the first {min(a.batched_cpu_mps, len(specs))} MPS of the population "
+                         f"({c_up} two-site updates), one gate at a time, torch CPU with {cores} "
+                         f"threads; a per-MPS rate, not extrapolated to the population"}
     chis = [int(bm.dims[j].max()) for j in range(len(specs))]
     out = {"workload": f"{len(specs)} MPS, qubits log-uniform 12-96, caps log-uniform 16-256, "
                        f"RY/RZ + CZ-ladder ansatz x {a.batched_layers} layers, complex64, "
@@ -459,7 +528,8 @@ def batched_bench(torch, a, dev):
            "shots": S * len(specs), "sample_s": t_samp,
            "shots_per_s": S * len(specs) / t_samp, "sampler": "grouped sweep, HIP-graph replay",
            "unbatched_reference": ref,
-           "speedup_vs_unbatched": (n2q / t_circ) / ref["updates_per_s"] if ref else None}
+           "speedup_vs_unbatched": (n2q / t_circ) / ref["updates_per_s"] if ref else None,
+           "cpu_baseline": cpu}
     return out
 
 
@@ -634,24 +704,9 @@ def main():
     # ---------------------------------------------------------------- CPU baseline
     cpu = None
     if not a.no_cpu and a.cpu_updates > 0:
-        from oracle import ref_mps as R
-        cores = len(os.sched_getaffinity(0))
-        torch.set_num_threads(cores)
-        touched = 2 + max(b for b, _ in sched[: 1 + a.cpu_updates])
-        om = R.OracleMPS(touched, bond_dim=1, eps_bond=1e-8, chi_max_per_bond=chi)
-        om.tensors = [t.cpu() for t in base[:touched]]
-        om.bond_dims = [chi] * (touched - 1)
-        b, U = sched[0]
-        om.apply_two_site_gate(b, torch.from_numpy(U).to(torch.complex64))  # warm-up
-        t0 = time.perf_counter()
-        for b, U in sched[1: 1 + a.cpu_updates]:
-            om.apply_two_site_gate(b, torch.from_numpy(U).to(torch.complex64))
-        t_cpu = time.perf_counter() - t0
-        cpu = {"value": a.cpu_updates / t_cpu, "unit": "updates/s", "cores": cores,
-               "kind": "port",
-               "sample": f"{a.cpu_updates} two-site updates (2048x2048 complex64 theta, "
-                         f"LAPACK SVD, rank rule, rebuild) on the same chi={chi} slice, "
-                         f"torch CPU with {cores} threads, after 1 warm-up update"}
+        cpu = cpu_slice_baseline(torch, base, sched, chi, a.cpu_updates,
+                                 f"{2 * chi}x{2 * chi} complex64 theta")
+    log("CPU baseline done")
 
     line = {
         "metric": METRIC, "value": ups, "unit": "updates/s", "n_gpus": 1, "steps": a.steps,
