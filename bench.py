@@ -102,6 +102,8 @@ def parse():
     ap.add_argument("--batched-cpu-mps", type=int, default=2,
                     help="MPS of the batched population run by the oracle on the host cores")
     ap.add_argument("--no-wide", action="store_true")
+    ap.add_argument("--no-peaks", action="store_true",
+                    help="skip the copy / rocBLAS GEMM reference rates")
     ap.add_argument("--wide-only", action="store_true")
     ap.add_argument("--batched-only", action="store_true")
     ap.add_argument("--mark", action="store_true",
@@ -354,6 +356,39 @@ def wide_bench(torch, a, dev):
     out["fidelity"] = fid
     out["max_abs_dF_fast_vs_plain"] = max((abs(f["fast"]["F"] - f["plain"]["F"]) for f in fid),
                                          default=None)
+    return out
+
+
+def measured_peaks(torch, dev):
+    """Reference rates measured on this GPU (context for the roofline
+    fractions, which use the published peaks): a 2 GiB device-to-device copy
+    (read + write bytes), and rocBLAS FP64 / FP32 GEMMs at 8192^3, each the
+    best of 3 after a warm-up, timed with HIP events."""
+    def best(fn, reps=3):
+        fn()
+        torch.cuda.synchronize()
+        t = []
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            torch.cuda.synchronize()
+            t.append(e0.elapsed_time(e1) * 1e-3)
+        return min(t)
+    src = torch.empty(1 << 28, dtype=torch.float64, device=dev)
+    dst = torch.empty_like(src)
+    t_copy = best(lambda: dst.copy_(src))
+    out = {"hbm_copy_GBps": 2 * src.numel() * 8 / t_copy / 1e9}
+    del src, dst
+    n = 8192
+    for name, dt in (("fp64_gemm_TFLOPs", torch.float64), ("fp32_gemm_TFLOPs", torch.float32)):
+        A = torch.randn(n, n, dtype=dt, device=dev)
+        B = torch.randn(n, n, dtype=dt, device=dev)
+        t = best(lambda: torch.matmul(A, B))
+        out[name] = 2.0 * n ** 3 / t / 1e12
+        del A, B
+    out["published"] = {"hbm_GBps": 8000.0, "fp64_mfma_TFLOPs": 78.6, "fp32_mfma_TFLOPs": 157.3}
     return out
 
 
@@ -701,6 +736,8 @@ def main():
     wide = None if a.no_wide else wide_bench(torch, a, dev)
     log("learned-rank form (chi 2048) done")
 
+    peaks = None if a.no_peaks else measured_peaks(torch, dev)
+    log("measured peaks done")
     # ---------------------------------------------------------------- CPU baseline
     cpu = None
     if not a.no_cpu and a.cpu_updates > 0:
@@ -735,6 +772,7 @@ def main():
         "sampler": samp,
         "batched": bat,
         "learned_rank_form": wide,
+        "measured_peaks": peaks,
     }
     print(json.dumps(line), flush=True)
 
