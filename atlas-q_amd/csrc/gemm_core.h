@@ -91,6 +91,24 @@ template <> struct TileK<float> { static constexpr int value = 16; };
 template <> struct TileK<double> { static constexpr int value = 8; };
 
 // ---------------------------------------------------------------- operands
+// One complex element through a global (address space 1) pointer: operand
+// pointers that come from descriptors in memory are otherwise generic, and
+// generic (flat) loads also count against the LDS counter, so every wait for
+// an LDS operand would wait for the next tile's global loads as well.
+template <typename T> struct Vec2;
+template <> struct Vec2<float> { typedef float type __attribute__((ext_vector_type(2))); };
+template <> struct Vec2<double> { typedef double type __attribute__((ext_vector_type(2))); };
+template <typename T> using v2_t = typename Vec2<T>::type;
+
+template <typename T>
+__device__ __forceinline__ v2_t<T> gload2(const c2_t<T>* p) {
+  return *(const __attribute__((address_space(1))) v2_t<T>*)(p);
+}
+
+// Operand functors: operator() is the bounds-checked element read; base(),
+// kstep(), conj_flag() and ones() give the mainloop's split access -- a
+// per-row (A) or per-column (B) pointer clamped into the operand (base_of),
+// advanced by k * kstep(), so every load is in bounds and unconditional.
 // A(m, k) = p[m*s0 + k*s1]
 template <typename T> struct AStrided {
   const c2_t<T>* p;
@@ -106,6 +124,13 @@ template <typename T> struct AStrided {
     }
     return v;
   }
+  __device__ __forceinline__ const c2_t<T>* base_of(int64_t m, bool& ok) const {
+    ok = m < M;
+    return p + (ok ? m : 0) * s0;
+  }
+  __device__ __forceinline__ int64_t kstep() const { return s1; }
+  __device__ __forceinline__ int conj_flag() const { return conj; }
+  __device__ __forceinline__ bool ones() const { return false; }
 };
 
 // Sampler left states: A(m, k) = base[m*ld + off[m] + k]; base == nullptr means
@@ -126,6 +151,16 @@ template <typename T> struct AGather {
     }
     return v;
   }
+  // base == nullptr (ones()): no load is made, the element is 1 for k < K
+  __device__ __forceinline__ const c2_t<T>* base_of(int64_t m, bool& ok) const {
+    ok = m < M;
+    if (base == nullptr) return nullptr;
+    const int64_t mc = ok ? m : 0;
+    return base + mc * ld + (off ? off[mc] : 0);
+  }
+  __device__ __forceinline__ int64_t kstep() const { return 1; }
+  __device__ __forceinline__ int conj_flag() const { return 0; }
+  __device__ __forceinline__ bool ones() const { return base == nullptr; }
 };
 
 // B(k, n) = p[k*s0 + n*s1]
@@ -143,6 +178,12 @@ template <typename T> struct BStrided {
     }
     return v;
   }
+  __device__ __forceinline__ const c2_t<T>* base_of(int64_t n, bool& ok) const {
+    ok = n < N;
+    return p + (ok ? n : 0) * s1;
+  }
+  __device__ __forceinline__ int64_t kstep() const { return s0; }
+  __device__ __forceinline__ int conj_flag() const { return conj; }
 };
 
 // Column map of a tile: PAIRED -> half h, column jj maps to h*half_n + j0 + jj
@@ -171,19 +212,23 @@ template <typename T> struct Frag {
   typename Mfma<T>::acc_t p3[RB][JB][2];   // 3M form only
 };
 
-template <typename T> struct alignas(16) Smem {
+// Stages = 1: one LDS tile, two barriers per k step (store, then read).
+// Stages = 2: a two-tile LDS ring, one barrier per k step -- the next tile is
+// stored into the other slot while this one is read (twice the LDS).
+template <typename T, int Stages = 1> struct alignas(16) Smem {
   static constexpr int BK = TileK<T>::value;
   static constexpr int PAD = 4;
-  T a[2][BK][kBM + PAD];
-  T b[2][BK][kBN + PAD];
+  T a[Stages][2][BK][kBM + PAD];
+  T b[Stages][2][BK][kBN + PAD];
 };
 
 // k_fast_a: the A functor is contiguous along k (else along m).
 // n_fast_b: the B functor is contiguous along n (else along k).
-template <typename T, bool KFastA, bool NFastB, bool ThreeM = false, class AOp, class BOp>
+template <typename T, bool KFastA, bool NFastB, bool ThreeM = false, class AOp, class BOp,
+          int Stages>
 __device__ __forceinline__ void cgemm_mainloop(const AOp& aop, const BOp& bop, const ColMap& cm,
-                                               int64_t K, int64_t m0, int64_t j0, Smem<T>& sm,
-                                               Frag<T>& fr) {
+                                               int64_t K, int64_t m0, int64_t j0,
+                                               Smem<T, Stages>& sm, Frag<T>& fr) {
   using M = Mfma<T>;
   constexpr int BK = Smem<T>::BK;
   constexpr int E = M::E;
@@ -207,8 +252,15 @@ __device__ __forceinline__ void cgemm_mainloop(const AOp& aop, const BOp& bop, c
           if (ThreeM) fr.p3[rb][jb][h][r] = 0;
         }
 
-  // global column of each B element this thread loads is loop-invariant
-  int64_t bcol[B_PER];
+  if (K <= 0) return;
+
+  // Per-thread operand pointers are loop-invariant.  The loads of a tile are
+  // unconditional (from addresses clamped into the operands) and the values
+  // are used only when the tile is stored to LDS, so the loads of the next
+  // tile issue back to back and stay in flight during this tile's MFMAs;
+  // out-of-range elements are zeroed and conjugation applied at the store.
+  const c2_t<T>* bptr[B_PER];
+  bool bok[B_PER];
   int bkk[B_PER], btc[B_PER];
 #pragma unroll
   for (int i = 0; i < B_PER; ++i) {
@@ -218,46 +270,60 @@ __device__ __forceinline__ void cgemm_mainloop(const AOp& aop, const BOp& bop, c
     else        { kk = e % BK;  tc = e / BK; }
     bkk[i] = kk;
     btc[i] = tc;
-    bcol[i] = cm.col(j0, tc / kBJ, tc % kBJ);
+    bptr[i] = bop.base_of(cm.col(j0, tc / kBJ, tc % kBJ), bok[i]);
   }
+  const c2_t<T>* aptr[A_PER];
+  bool aok[A_PER];
   int amm[A_PER], akk[A_PER];
 #pragma unroll
   for (int i = 0; i < A_PER; ++i) {
     int e = tid + i * kThreads;
     if (KFastA) { akk[i] = e % BK; amm[i] = e / BK; }
     else        { amm[i] = e % kBM; akk[i] = e / kBM; }
+    aptr[i] = aop.base_of(m0 + amm[i], aok[i]);
   }
+  const int64_t a_ks = aop.kstep(), b_ks = bop.kstep();
+  const bool a_ones = aop.ones();
+  const int a_conj = aop.conj_flag(), b_conj = bop.conj_flag();
 
-  c2_t<T> ra[A_PER], rbv[B_PER];
+  v2_t<T> ra[A_PER], rbv[B_PER];
+  unsigned amask = 0, bmask = 0;
   auto fetch = [&](int64_t k0) {
-#pragma unroll
-    for (int i = 0; i < A_PER; ++i) ra[i] = aop(m0 + amm[i], k0 + akk[i]);
-#pragma unroll
-    for (int i = 0; i < B_PER; ++i) {
-      if (bcol[i] != INT64_MAX) {
-        rbv[i] = bop(k0 + bkk[i], bcol[i]);
-      } else {
-        rbv[i].x = 0; rbv[i].y = 0;
-      }
-    }
-  };
-
-  fetch(0);
-  for (int64_t k0 = 0; k0 < K; k0 += BK) {
-    __syncthreads();
+    amask = 0;
+    bmask = 0;
 #pragma unroll
     for (int i = 0; i < A_PER; ++i) {
-      sm.a[0][akk[i]][amm[i]] = ra[i].x;
-      sm.a[1][akk[i]][amm[i]] = ra[i].y;
+      const int64_t k = k0 + akk[i];
+      amask |= (unsigned)(aok[i] && k < K) << i;
+      if (a_ones) {
+        ra[i].x = 1;
+        ra[i].y = 0;
+      } else {
+        ra[i] = gload2<T>(aptr[i] + (k < K ? k : 0) * a_ks);
+      }
     }
 #pragma unroll
     for (int i = 0; i < B_PER; ++i) {
-      sm.b[0][bkk[i]][btc[i]] = rbv[i].x;
-      sm.b[1][bkk[i]][btc[i]] = rbv[i].y;
+      const int64_t k = k0 + bkk[i];
+      bmask |= (unsigned)(bok[i] && k < K) << i;
+      rbv[i] = gload2<T>(bptr[i] + (k < K ? k : 0) * b_ks);
     }
-    __syncthreads();
-    if (k0 + BK < K) fetch(k0 + BK);  // overlap next tile's loads with MFMA
-
+  };
+  auto store = [&](int s) {
+#pragma unroll
+    for (int i = 0; i < A_PER; ++i) {
+      const bool ok = (amask >> i) & 1u;
+      sm.a[s][0][akk[i]][amm[i]] = ok ? ra[i].x : T(0);
+      sm.a[s][1][akk[i]][amm[i]] = ok ? (a_conj ? -ra[i].y : ra[i].y) : T(0);
+    }
+#pragma unroll
+    for (int i = 0; i < B_PER; ++i) {
+      const bool ok = (bmask >> i) & 1u;
+      sm.b[s][0][bkk[i]][btc[i]] = ok ? rbv[i].x : T(0);
+      sm.b[s][1][bkk[i]][btc[i]] = ok ? (b_conj ? -rbv[i].y : rbv[i].y) : T(0);
+    }
+  };
+  auto compute = [&](int s) {
 #pragma unroll
     for (int kk = 0; kk < BK; kk += M::TK) {
       const int kr = kk + M::op_k(lane);
@@ -265,8 +331,8 @@ __device__ __forceinline__ void cgemm_mainloop(const AOp& aop, const BOp& bop, c
 #pragma unroll
       for (int rb = 0; rb < Frag<T>::RB; ++rb) {
         const int row = 32 * wm + rb * E + M::a_row(lane);
-        ar[rb] = sm.a[0][kr][row];
-        ai[rb] = sm.a[1][kr][row];
+        ar[rb] = sm.a[s][0][kr][row];
+        ai[rb] = sm.a[s][1][kr][row];
         nai[rb] = -ai[rb];
         as[rb] = ar[rb] + ai[rb];
       }
@@ -275,8 +341,8 @@ __device__ __forceinline__ void cgemm_mainloop(const AOp& aop, const BOp& bop, c
 #pragma unroll
         for (int jb = 0; jb < Frag<T>::JB; ++jb) {
           const int col = h * kBJ + 32 * wn + jb * E + M::op_idx(lane);
-          const T br = sm.b[0][kr][col];
-          const T bi = sm.b[1][kr][col];
+          const T br = sm.b[s][0][kr][col];
+          const T bi = sm.b[s][1][kr][col];
           const T bs = br + bi;
 #pragma unroll
           for (int rb = 0; rb < Frag<T>::RB; ++rb) {
@@ -292,6 +358,32 @@ __device__ __forceinline__ void cgemm_mainloop(const AOp& aop, const BOp& bop, c
             }
           }
         }
+    }
+  };
+
+  if constexpr (Stages == 1) {
+    fetch(0);
+    for (int64_t k0 = 0; k0 < K; k0 += BK) {
+      __syncthreads();
+      store(0);
+      __syncthreads();
+      if (k0 + BK < K) fetch(k0 + BK);  // overlap next tile's loads with MFMA
+      compute(0);
+    }
+  } else {
+    // The caller has synchronised since the last read of either slot.
+    fetch(0);
+    store(0);
+    __syncthreads();
+    if (BK < K) fetch(BK);
+    int s = 0;
+    for (int64_t k0 = 0; k0 < K; k0 += BK) {
+      compute(s);
+      // slot s^1 was last read before the previous barrier
+      if (k0 + BK < K) store(s ^ 1);
+      __syncthreads();
+      if (k0 + 2 * BK < K) fetch(k0 + 2 * BK);
+      s ^= 1;
     }
   }
   if (ThreeM) {
