@@ -212,23 +212,21 @@ template <typename T> struct Frag {
   typename Mfma<T>::acc_t p3[RB][JB][2];   // 3M form only
 };
 
-// Stages = 1: one LDS tile, two barriers per k step (store, then read).
-// Stages = 2: a two-tile LDS ring, one barrier per k step -- the next tile is
-// stored into the other slot while this one is read (twice the LDS).
-template <typename T, int Stages = 1> struct alignas(16) Smem {
+// One LDS tile, two barriers per k step (a two-slot ring with one barrier
+// measured slower: the barrier is not what bounds the loop).
+template <typename T> struct alignas(16) Smem {
   static constexpr int BK = TileK<T>::value;
   static constexpr int PAD = 4;
-  T a[Stages][2][BK][kBM + PAD];
-  T b[Stages][2][BK][kBN + PAD];
+  T a[2][BK][kBM + PAD];
+  T b[2][BK][kBN + PAD];
 };
 
 // k_fast_a: the A functor is contiguous along k (else along m).
 // n_fast_b: the B functor is contiguous along n (else along k).
-template <typename T, bool KFastA, bool NFastB, bool ThreeM = false, class AOp, class BOp,
-          int Stages>
+template <typename T, bool KFastA, bool NFastB, bool ThreeM = false, class AOp, class BOp>
 __device__ __forceinline__ void cgemm_mainloop(const AOp& aop, const BOp& bop, const ColMap& cm,
-                                               int64_t K, int64_t m0, int64_t j0,
-                                               Smem<T, Stages>& sm, Frag<T>& fr) {
+                                               int64_t K, int64_t m0, int64_t j0, Smem<T>& sm,
+                                               Frag<T>& fr) {
   using M = Mfma<T>;
   constexpr int BK = Smem<T>::BK;
   constexpr int E = M::E;
@@ -309,21 +307,21 @@ __device__ __forceinline__ void cgemm_mainloop(const AOp& aop, const BOp& bop, c
       rbv[i] = gload2<T>(bptr[i] + (k < K ? k : 0) * b_ks);
     }
   };
-  auto store = [&](int s) {
+  auto store = [&]() {
 #pragma unroll
     for (int i = 0; i < A_PER; ++i) {
       const bool ok = (amask >> i) & 1u;
-      sm.a[s][0][akk[i]][amm[i]] = ok ? ra[i].x : T(0);
-      sm.a[s][1][akk[i]][amm[i]] = ok ? (a_conj ? -ra[i].y : ra[i].y) : T(0);
+      sm.a[0][akk[i]][amm[i]] = ok ? ra[i].x : T(0);
+      sm.a[1][akk[i]][amm[i]] = ok ? (a_conj ? -ra[i].y : ra[i].y) : T(0);
     }
 #pragma unroll
     for (int i = 0; i < B_PER; ++i) {
       const bool ok = (bmask >> i) & 1u;
-      sm.b[s][0][bkk[i]][btc[i]] = ok ? rbv[i].x : T(0);
-      sm.b[s][1][bkk[i]][btc[i]] = ok ? (b_conj ? -rbv[i].y : rbv[i].y) : T(0);
+      sm.b[0][bkk[i]][btc[i]] = ok ? rbv[i].x : T(0);
+      sm.b[1][bkk[i]][btc[i]] = ok ? (b_conj ? -rbv[i].y : rbv[i].y) : T(0);
     }
   };
-  auto compute = [&](int s) {
+  auto compute = [&]() {
 #pragma unroll
     for (int kk = 0; kk < BK; kk += M::TK) {
       const int kr = kk + M::op_k(lane);
@@ -331,8 +329,8 @@ __device__ __forceinline__ void cgemm_mainloop(const AOp& aop, const BOp& bop, c
 #pragma unroll
       for (int rb = 0; rb < Frag<T>::RB; ++rb) {
         const int row = 32 * wm + rb * E + M::a_row(lane);
-        ar[rb] = sm.a[s][0][kr][row];
-        ai[rb] = sm.a[s][1][kr][row];
+        ar[rb] = sm.a[0][kr][row];
+        ai[rb] = sm.a[1][kr][row];
         nai[rb] = -ai[rb];
         as[rb] = ar[rb] + ai[rb];
       }
@@ -341,8 +339,8 @@ __device__ __forceinline__ void cgemm_mainloop(const AOp& aop, const BOp& bop, c
 #pragma unroll
         for (int jb = 0; jb < Frag<T>::JB; ++jb) {
           const int col = h * kBJ + 32 * wn + jb * E + M::op_idx(lane);
-          const T br = sm.b[s][0][kr][col];
-          const T bi = sm.b[s][1][kr][col];
+          const T br = sm.b[0][kr][col];
+          const T bi = sm.b[1][kr][col];
           const T bs = br + bi;
 #pragma unroll
           for (int rb = 0; rb < Frag<T>::RB; ++rb) {
@@ -361,30 +359,13 @@ __device__ __forceinline__ void cgemm_mainloop(const AOp& aop, const BOp& bop, c
     }
   };
 
-  if constexpr (Stages == 1) {
-    fetch(0);
-    for (int64_t k0 = 0; k0 < K; k0 += BK) {
-      __syncthreads();
-      store(0);
-      __syncthreads();
-      if (k0 + BK < K) fetch(k0 + BK);  // overlap next tile's loads with MFMA
-      compute(0);
-    }
-  } else {
-    // The caller has synchronised since the last read of either slot.
-    fetch(0);
-    store(0);
+  fetch(0);
+  for (int64_t k0 = 0; k0 < K; k0 += BK) {
     __syncthreads();
-    if (BK < K) fetch(BK);
-    int s = 0;
-    for (int64_t k0 = 0; k0 < K; k0 += BK) {
-      compute(s);
-      // slot s^1 was last read before the previous barrier
-      if (k0 + BK < K) store(s ^ 1);
-      __syncthreads();
-      if (k0 + 2 * BK < K) fetch(k0 + 2 * BK);
-      s ^= 1;
-    }
+    store();
+    __syncthreads();
+    if (k0 + BK < K) fetch(k0 + BK);  // in flight during this tile's MFMAs
+    compute();
   }
   if (ThreeM) {
     // Cr = P1 - P2, Ci = P3 - P1 - P2
