@@ -205,6 +205,38 @@ def test_grouped_herk_lower_triangle(M, N, xcd):
         assert torch.allclose(g[low], ref[low], atol=1e-10 * N)
 
 
+@pytest.mark.parametrize("dtype", [torch.complex64, torch.complex128])
+@pytest.mark.parametrize("b_layout", ["row", "col"])
+def test_grouped_gemm_mixed_problems(dtype, b_layout):
+    """One grouped launch over problems of unrelated shapes (ragged M, N, K,
+    K below one k tile, K = 1), per-problem conjugation flags, and A read with
+    a negative row stride (rows reversed, as the projection reads the
+    eigensolver's output): every element matches the float64 product."""
+    shapes = [(1, 1, 1), (67, 3, 130), (130, 33, 257), (64, 16, 128), (5, 200, 9)]
+    flags = [(False, False), (True, False), (False, True), (True, True), (True, False)]
+    g = torch.Generator().manual_seed(11 + (b_layout == "col"))
+    descs, refs, outs, keep = [], [], [], []
+    for (M, K_, N), (ca, cb) in zip(shapes, flags):
+        A = _rand((M, K_), dtype, g).to(DEV)
+        Bm = _rand((K_, N), dtype, g)
+        if b_layout == "col":                  # column-major, column stride K + 1
+            B = torch.zeros(N * (K_ + 1), dtype=dtype, device=DEV).as_strided((K_, N), (1, K_ + 1))
+            B.copy_(Bm.to(DEV))
+        else:
+            B = Bm.to(DEV)
+        C = torch.full((M, N), float("nan"), dtype=dtype, device=DEV)
+        Af = A.flip(0).contiguous()            # A = rows of Af read bottom-up
+        descs.append(K.gemm_desc(Af, B, C, conj_a=ca, conj_b=cb, M=M, K=K_, N=N,
+                                 a_ptr=Af[M - 1].data_ptr(), a_strides=(-K_, 1)))
+        A64, B64 = A.cpu().to(torch.complex128), Bm.to(torch.complex128)
+        refs.append((A64.conj() if ca else A64) @ (B64.conj() if cb else B64))
+        outs.append(C)
+        keep += [A, Af, B]
+    K.grouped_gemm_descs(np.stack(descs), dtype, DEV)
+    for C, ref in zip(outs, refs):
+        assert _relerr(C, ref) < 10 * TOL[dtype]
+
+
 def test_heevd_reads_only_the_lower_triangle():
     """The row-major lower triangle is all aq_heevd_batched reads (column-major
     upper): the other triangle may hold anything."""
