@@ -223,6 +223,8 @@ template <typename T> struct alignas(16) Smem {
 
 // k_fast_a: the A functor is contiguous along k (else along m).
 // n_fast_b: the B functor is contiguous along n (else along k).
+// K is the reduction length the functors were built with: element k is
+// in range when k < K (rows / columns are checked by the functors).
 template <typename T, bool KFastA, bool NFastB, bool ThreeM = false, class AOp, class BOp>
 __device__ __forceinline__ void cgemm_mainloop(const AOp& aop, const BOp& bop, const ColMap& cm,
                                                int64_t K, int64_t m0, int64_t j0, Smem<T>& sm,
