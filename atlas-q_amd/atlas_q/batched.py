@@ -51,8 +51,18 @@ def _stream() -> ctypes.c_void_p:
 
 
 def _h2d(arr: np.ndarray, device) -> torch.Tensor:
-    """Host array -> device tensor (a synchronous copy of a private host copy)."""
-    return torch.from_numpy(np.ascontiguousarray(arr).copy()).to(device)
+    """Host array -> device tensor, queued on the current stream without
+    waiting for it: the values are staged in page-locked memory now (so the
+    caller may reuse ``arr``) and copied asynchronously; torch's pinned-memory
+    cache keeps the staging block until the copy has run.  (A copy from
+    pageable memory would synchronise the stream, draining the GPU's queue at
+    every upload.)"""
+    src = torch.from_numpy(np.ascontiguousarray(arr))
+    if torch.device(device).type == "cpu":
+        return src.clone()
+    staged = torch.empty(src.shape, dtype=src.dtype, pin_memory=True)
+    staged.copy_(src)
+    return staged.to(device, non_blocking=True)
 
 
 def _upload(arr: np.ndarray, device) -> torch.Tensor:
@@ -139,7 +149,7 @@ def _condition_bucket(T: torch.Tensor, orders: np.ndarray, jitter: torch.Tensor)
     d = torch.where(d > 0, d, torch.ones_like(d))
     if int(orders.min()) < sz:
         i = torch.arange(sz, device=T.device, dtype=torch.float64)
-        pad = torch.from_numpy(np.arange(sz)[None, :] >= orders[:, None]).to(T.device)
+        pad = _h2d(np.arange(sz)[None, :] >= orders[:, None], T.device)
         dg.copy_(torch.where(pad, (-d * (1.0 + i / sz)).to(T.dtype), dg))
     if jitter is not None:
         T.add_(jitter.unsqueeze(0) * (1e-13 * d).unsqueeze(-1).to(T.dtype))
