@@ -166,8 +166,12 @@ class BatchedMPS:
     """Population of MPS (n_j qubits, cap chi_j) starting in |0...0>."""
 
     def __init__(self, specs: Sequence[Tuple[int, int]], eps_bond: float = 1e-8,
-                 device="cuda", autotuner: Autotuner = None):
+                 device="cuda", autotuner: Autotuner = None, eig_workers: int = 3):
         self.dev = torch.device(device)
+        # host threads (one HIP stream each) for the rocSOLVER size buckets of a
+        # sub-layer: each call is a chain of small per-column launches, so
+        # buckets solved concurrently overlap their launch latencies
+        self.eig_workers = int(eig_workers)
         if self.dev.type != "cuda":
             raise RuntimeError("BatchedMPS runs on a HIP device only")
         _ffi.check_layouts()
@@ -333,17 +337,51 @@ class BatchedMPS:
         self._grouped_gemm(_ffi.AQ_C128, gram[left], 1 | 4, "gram")
         self._grouped_gemm(_ffi.AQ_C128, gram[~left], 2 | 4, "gram")
         # ---- eigen-decompositions per bucket (descending); success flags stay
-        # on the device and come back with the rank results (one sync)
-        W_all, V_all, oks = {}, {}, []
+        # on the device and come back with the rank results (one sync).  The
+        # rocSOLVER buckets run on worker threads with their own streams
+        # while the Jacobi buckets (one launch each) run on this one.
+        big = [sz for sz in buckets if sz > JACOBI_MAX]
+        for sz in big:
+            if sz not in self._jitter:
+                self._jitter[sz] = _jitter_matrix(sz, self.dev)
+
+        def solve_big(sz):
+            idx, T = buckets[sz]
+            _condition_bucket(T, s[idx], self._jitter[sz])
+            return heevd_batched(T, inplace=True, return_ok="device")
+
+        main = torch.cuda.current_stream(self.dev)
+        futs = {}
+        if self.eig_workers > 1 and len(big) > 1:
+            from .adaptive_mps import _thread_stream, _worker_pool
+            ready = main.record_event()
+
+            def job(sz):
+                st = _thread_stream(self.dev)
+                with torch.cuda.stream(st):
+                    st.wait_event(ready)
+                    out = solve_big(sz)
+                    done = st.record_event()
+                return out, done
+
+            pool = _worker_pool(self.eig_workers)
+            futs = {sz: pool.submit(job, sz) for sz in big}
+        res = {}
         for sz, (idx, T) in buckets.items():
-            if sz > JACOBI_MAX:
-                if sz not in self._jitter:
-                    self._jitter[sz] = _jitter_matrix(sz, self.dev)
-                _condition_bucket(T, s[idx], self._jitter[sz])
-                W, V, ok = heevd_batched(T, inplace=True, return_ok="device")
-            else:
+            if sz <= JACOBI_MAX:
                 W, V = heev_small(T, orders=_h2d(s[idx].astype(np.int64), self.dev))
-                ok = finite_rows(W, V)
+                res[sz] = (W, V, finite_rows(W, V))
+            elif not futs:
+                res[sz] = solve_big(sz)
+        for sz, f in futs.items():
+            out, done = f.result()
+            main.wait_event(done)
+            for t in out:                 # produced on a worker stream, read on this one
+                t.record_stream(main)
+            res[sz] = out
+        W_all, V_all, oks = {}, {}, []
+        for sz in buckets:
+            W, V, ok = res[sz]
             W_all[sz], V_all[sz] = W.contiguous(), V.contiguous()
             oks.append(ok)
         soff, _ = _prefix(s)
