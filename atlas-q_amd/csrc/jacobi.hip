@@ -110,6 +110,24 @@ __global__ __launch_bounds__(NT) void heev_jacobi_kernel(int n, const int64_t* _
   const double thresh2 = tol * tol * fro;
   int sweep = 0;
   const int half = ne / 2;
+  // this thread's items of the A pass (blocks kr, kc) and of the V pass
+  // (pair k, row r) are the same in every round: decompose them once
+  // (integer division by the runtime order costs tens of instructions)
+  constexpr int AIT = (JM / 2 * (JM / 2) + NT - 1) / NT;
+  constexpr int VIT = (JM * (JM / 2) + NT - 1) / NT;
+  int akr[AIT], akc[AIT], vk[VIT], vr[VIT];
+#pragma unroll
+  for (int j = 0; j < AIT; ++j) {
+    const int it = tid + j * NT;
+    akr[j] = it < half * half ? it / half : -1;
+    akc[j] = it < half * half ? it % half : 0;
+  }
+#pragma unroll
+  for (int j = 0; j < VIT; ++j) {
+    const int it = tid + j * NT;
+    vk[j] = it < ne * half ? it / ne : -1;
+    vr[j] = it < ne * half ? it % ne : 0;
+  }
   for (; sweep < max_sweeps; ++sweep) {
     double off = 0.0;
     for (int i = tid; i < ne * ne; i += blockDim.x) {
@@ -126,10 +144,16 @@ __global__ __launch_bounds__(NT) void heev_jacobi_kernel(int n, const int64_t* _
       // tournament, position 0 fixed and the others rotated by `round`:
       // pos(0) = 0, pos(i) = 1 + ((i - 1 - round) mod (ne - 1))
       if (tid < half) {
+        // tid - 1 - round and i2 - 1 - round lie in (-m1, m1): one
+        // conditional add is the modulo
         const int m1 = ne - 1;
         const int i2 = ne - 1 - tid;
-        int p = tid == 0 ? 0 : 1 + ((tid - 1 - round) % m1 + m1) % m1;
-        int q = 1 + ((i2 - 1 - round) % m1 + m1) % m1;
+        int pp = tid - 1 - round;
+        pp += pp < 0 ? m1 : 0;
+        int qq = i2 - 1 - round;
+        qq += qq < 0 ? m1 : 0;
+        int p = tid == 0 ? 0 : 1 + pp;
+        int q = 1 + qq;
         if (p > q) { const int t = p; p = q; q = t; }
         sm.pp[tid] = p;
         sm.pq[tid] = q;
@@ -163,8 +187,10 @@ __global__ __launch_bounds__(NT) void heev_jacobi_kernel(int n, const int64_t* _
       // pairs, so block (pair kr, pair kc) -- rows {p_r, q_r} x columns
       // {p_c, q_c} -- maps to J_r^H A[block] J_c from its own old values
       // (one pass over A, no intermediate barrier)
-      for (int it = tid; it < half * half; it += blockDim.x) {
-        const int kr = it / half, kc = it % half;
+#pragma unroll
+      for (int j = 0; j < AIT; ++j) {
+        const int kr = akr[j], kc = akc[j];
+        if (kr < 0) continue;
         const int pr = sm.pp[kr], qr = sm.pq[kr], pc = sm.pp[kc], qc = sm.pq[kc];
         const double cc = sm.cs[kc];
         const double sc = sm.sp[kc].x * sm.ep[kc].x + sm.sp[kc].y * sm.ep[kc].y;
@@ -198,8 +224,10 @@ __global__ __launch_bounds__(NT) void heev_jacobi_kernel(int n, const int64_t* _
         sm.a[qr][qc] = n11;
       }
       // V <- V J   (item = row r, pair k; V is not read by the A pass)
-      for (int it = tid; it < ne * half; it += blockDim.x) {
-        const int k = it / ne, r = it % ne;
+#pragma unroll
+      for (int j = 0; j < VIT; ++j) {
+        const int k = vk[j], r = vr[j];
+        if (k < 0) continue;
         const int p = sm.pp[k], q = sm.pq[k];
         const double c = sm.cs[k];
         const double s = sm.sp[k].x * sm.ep[k].x + sm.sp[k].y * sm.ep[k].y;  // |s|, sign kept
