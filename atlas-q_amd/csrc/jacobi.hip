@@ -113,14 +113,22 @@ __global__ __launch_bounds__(NT) void heev_jacobi_kernel(int n, const int64_t* _
   // this thread's items of the A pass (blocks kr, kc) and of the V pass
   // (pair k, row r) are the same in every round: decompose them once
   // (integer division by the runtime order costs tens of instructions)
-  constexpr int AIT = (JM / 2 * (JM / 2) + NT - 1) / NT;
+  // A pass: the blocks (kr, kc) with kr >= kc, numbered row by row
+  constexpr int AIT = (JM / 2 * (JM / 2 + 1) / 2 + NT - 1) / NT;
   constexpr int VIT = (JM * (JM / 2) + NT - 1) / NT;
   int akr[AIT], akc[AIT], vk[VIT], vr[VIT];
 #pragma unroll
   for (int j = 0; j < AIT; ++j) {
     const int it = tid + j * NT;
-    akr[j] = it < half * half ? it / half : -1;
-    akc[j] = it < half * half ? it % half : 0;
+    akr[j] = -1;
+    akc[j] = 0;
+    if (it < half * (half + 1) / 2) {
+      int r = (int)((sqrt(8.0 * it + 1.0) - 1.0) * 0.5);
+      while (r * (r + 1) / 2 > it) --r;
+      while ((r + 1) * (r + 2) / 2 <= it) ++r;
+      akr[j] = r;
+      akc[j] = it - r * (r + 1) / 2;
+    }
   }
 #pragma unroll
   for (int j = 0; j < VIT; ++j) {
@@ -186,7 +194,9 @@ __global__ __launch_bounds__(NT) void heev_jacobi_kernel(int n, const int64_t* _
       // A <- J^H A J by 2 x 2 blocks: the rotations of a round act on disjoint
       // pairs, so block (pair kr, pair kc) -- rows {p_r, q_r} x columns
       // {p_c, q_c} -- maps to J_r^H A[block] J_c from its own old values
-      // (one pass over A, no intermediate barrier)
+      // (one pass over A, no intermediate barrier).  Only blocks with
+      // kr >= kc are computed; block (kc, kr) is written as the conjugate
+      // transpose, which keeps A exactly Hermitian.
 #pragma unroll
       for (int j = 0; j < AIT; ++j) {
         const int kr = akr[j], kc = akc[j];
@@ -222,6 +232,12 @@ __global__ __launch_bounds__(NT) void heev_jacobi_kernel(int n, const int64_t* _
         sm.a[pr][qc] = n01;
         sm.a[qr][pc] = n10;
         sm.a[qr][qc] = n11;
+        if (kr != kc) {
+          sm.a[pc][pr] = cconj(n00);
+          sm.a[qc][pr] = cconj(n01);
+          sm.a[pc][qr] = cconj(n10);
+          sm.a[qc][qr] = cconj(n11);
+        }
       }
       // V <- V J   (item = row r, pair k; V is not read by the A pass)
 #pragma unroll
