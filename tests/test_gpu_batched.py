@@ -223,3 +223,24 @@ def test_batched_padded_buckets_match_single_mps_gram():
             for q in range(1, n):
                 r = r @ t[q][:, bits[q], :].to(torch.complex128)
             assert abs(complex(r[0, 0]) - ref.get_amplitude(bstr)) < 1e-5
+
+
+def test_concurrent_bucket_eigensolves_match_one_stream():
+    """The rocSOLVER size buckets of a sub-layer solved on three worker
+    streams (each waiting for the Gram launch, the caller's stream waiting
+    for each of them) give bit-identical states and logs to one stream."""
+    specs = [(16, 256), (14, 128), (13, 50)]
+    circs = [hea(n, 9, 500 + j) for j, (n, _) in enumerate(specs)]
+    runs = []
+    for workers in (1, 3):
+        bm = BatchedMPS(specs, eps_bond=1e-8, device=DEV, autotuner=_tuner(),
+                        eig_workers=workers)
+        bm.run_hea(circs)
+        torch.cuda.synchronize()
+        runs.append((bm.arena.clone(), {k: list(v) for k, v in bm.logs.items()},
+                     bm.eig_fallbacks, max(int(bm.dims[j].max()) for j in range(len(specs)))))
+    (a1, l1, f1, d1), (a3, l3, f3, d3) = runs
+    assert d1 > 64 and d1 == d3          # rocSOLVER buckets were used
+    assert f1 == f3
+    assert l1 == l3
+    assert torch.equal(a1, a3)
