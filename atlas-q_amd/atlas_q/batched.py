@@ -636,6 +636,9 @@ class BatchedMPS:
             self._site_lists.append((_upload(d, self.dev), _h2d(tpre, self.dev),
                                      _h2d(spre, self.dev), len(js), ttot, stot))
         self._grid_sample = self.tuner.grid("sample", self._site_lists[0][4], None)
+        # the descriptors are on the device when this returns, so the sweep may
+        # be launched (or captured) on any stream
+        torch.cuda.current_stream(self.dev).synchronize()
 
     def sweep(self):
         """The full sweep for all MPS (launches only; uniforms must be filled)."""
