@@ -115,6 +115,10 @@ def parse():
     ap.add_argument("--gemm-probe", action="store_true",
                     help="only run 5 FP64 Gram GEMMs X X^H of the bench's 2048 x 2048 theta "
                          "(the dominant hand-written kernel; for counter runs) and exit")
+    ap.add_argument("--eig-probe", action="store_true",
+                    help="only time rocSOLVER's strided-batched Hermitian eigensolvers on Gram "
+                         "matrices (the variants, the batch-size scan at the bench's order and "
+                         "concurrent calls) and exit")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters for the traffic figure) and exit")
@@ -593,6 +597,166 @@ def traffic_from_profile(kernel: str):
     return None, None
 
 
+def eig_probe(torch, a):
+    """rocSOLVER's strided-batched Hermitian eigensolvers (the rocSOLVER the
+    library links) on copies of one Gram matrix G = X X^H, X complex Gaussian:
+    (1) zheevd vs zheevdj vs zheevj (batch 64; orders of the batched form's
+    buckets) and at order 2 chi the tridiagonalisation alone (zhetrd), the
+    partial-spectrum zheevdx (top half) and complex64 cheevd; (2) zheevd vs
+    batch size at order 2 chi; (3) 64 matrices as 1, 2, 4 or 8 concurrent
+    calls, one host thread, rocBLAS handle and stream each.  HIP events on the
+    calls' stream, best of the timed repetitions after a warm-up call."""
+    import ctypes
+    import threading
+    from atlas_q._ffi import lib
+    lib()                                               # loads the library's rocSOLVER
+    rb = ctypes.CDLL("librocblas.so.5")
+    rs = ctypes.CDLL("librocsolver.so.0")
+    vp, i32, i64, f64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_double
+    FILL_UPPER, EVECT, ERANGE_INDEX, ESORT_ASC = 121, 211, 233, 252
+
+    def handle(stream):
+        h = vp()
+        assert rb.rocblas_create_handle(ctypes.byref(h)) == 0
+        assert rb.rocblas_set_stream(h, vp(stream.cuda_stream)) == 0
+        return h
+
+    dev = torch.device("cuda", 0)
+    gen = torch.Generator(device=dev).manual_seed(a.seed)
+    p = lambda t: vp(t.data_ptr())
+
+    def gram(n):
+        X = torch.randn((n, n), dtype=torch.complex128, device=dev, generator=gen)
+        return X @ X.conj().T
+
+    def call(h, meth, n, A, b, W, E, info, aux):
+        nn = n * n
+        if meth == "zheevd":
+            r = rs.rocsolver_zheevd_strided_batched(h, i32(EVECT), i32(FILL_UPPER), i32(n), p(A),
+                                                    i32(n), i64(nn), p(W), i64(n), p(E), i64(n),
+                                                    p(info), i32(b))
+        elif meth == "zheevdj":
+            r = rs.rocsolver_zheevdj_strided_batched(h, i32(EVECT), i32(FILL_UPPER), i32(n), p(A),
+                                                     i32(n), i64(nn), p(W), i64(n), p(info), i32(b))
+        elif meth == "zheevj":
+            r = rs.rocsolver_zheevj_strided_batched(h, i32(ESORT_ASC), i32(EVECT), i32(FILL_UPPER),
+                                                    i32(n), p(A), i32(n), i64(nn), f64(0.0),
+                                                    p(aux["res"]), i32(100), p(aux["sweeps"]),
+                                                    p(W), i64(n), p(info), i32(b))
+        elif meth == "zhetrd":
+            r = rs.rocsolver_zhetrd_strided_batched(h, i32(FILL_UPPER), i32(n), p(A), i32(n),
+                                                    i64(nn), p(W), i64(n), p(E), i64(n),
+                                                    p(aux["tau"]), i64(n), i32(b))
+        elif meth == "zheevdx":
+            r = rs.rocsolver_zheevdx_strided_batched(h, i32(EVECT), i32(ERANGE_INDEX),
+                                                     i32(FILL_UPPER), i32(n), p(A), i32(n),
+                                                     i64(nn), f64(0.0), f64(0.0), i32(n // 2 + 1),
+                                                     i32(n), p(aux["nev"]), p(W), i64(n),
+                                                     p(aux["Z"]), i32(n), i64(nn), p(info), i32(b))
+        elif meth == "cheevd":
+            r = rs.rocsolver_cheevd_strided_batched(h, i32(EVECT), i32(FILL_UPPER), i32(n),
+                                                    p(aux["A32"]), i32(n), i64(nn), p(aux["W32"]),
+                                                    i64(n), p(aux["E32"]), i64(n), p(info), i32(b))
+        assert r == 0, (meth, r)
+
+    def timed(meth, n, G, b, reps):
+        st = torch.cuda.current_stream(dev)
+        h = handle(st)
+        A = torch.empty((b, n, n), dtype=torch.complex128, device=dev)
+        W = torch.empty((b, n), dtype=torch.float64, device=dev)
+        E = torch.empty((b, n), dtype=torch.float64, device=dev)
+        info = torch.zeros(b, dtype=torch.int32, device=dev)
+        aux = {"res": torch.empty(b, dtype=torch.float64, device=dev),
+               "sweeps": torch.zeros(b, dtype=torch.int32, device=dev),
+               "tau": torch.empty((b, n), dtype=torch.complex128, device=dev),
+               "nev": torch.zeros(b, dtype=torch.int32, device=dev)}
+        if meth == "zheevdx":
+            aux["Z"] = torch.empty((b, n, n), dtype=torch.complex128, device=dev)
+        if meth == "cheevd":
+            aux["A32"] = torch.empty((b, n, n), dtype=torch.complex64, device=dev)
+            aux["W32"] = torch.empty((b, n), dtype=torch.float32, device=dev)
+            aux["E32"] = torch.empty((b, n), dtype=torch.float32, device=dev)
+        best = float("inf")
+        for rep in range(reps + 1):
+            A.copy_(G.expand(b, n, n))
+            if meth == "cheevd":
+                aux["A32"].copy_(A)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            call(h, meth, n, A, b, W, E, info, aux)
+            e1.record()
+            torch.cuda.synchronize()
+            if rep:
+                best = min(best, e0.elapsed_time(e1))
+        rb.rocblas_destroy_handle(h)
+        w = (aux["W32"].double() if meth == "cheevd" else W)[-1]
+        return best, int((info != 0).sum()), w
+
+    out = {"variants": [], "batch_scan": [], "concurrent": []}
+    n2 = 2 * a.chi
+    for n in (128, 256, 512, n2):
+        G = gram(n)
+        ref = None
+        meths = (("zheevd", 64), ("zheevdj", 64), ("zheevj", 64)) if n <= 512 else \
+            (("zheevd", 64), ("zheevdj", 64), ("zhetrd", 64), ("cheevd", 64), ("zheevdx", 8))
+        for meth, b in meths:
+            ms, bad, w = timed(meth, n, G, b, 1 if ms_slow(meth, n) else 2)
+            if meth == "zheevd":
+                ref = w
+            dw = None if meth == "zhetrd" else float((w.sort().values - ref.sort().values).abs().max()
+                                                     / ref.abs().max()) if meth != "zheevdx" else None
+            row = {"method": meth, "n": n, "batch": b, "ms": ms, "ms_per_matrix": ms / b,
+                   "info_nonzero": bad, "max_rel_dW_vs_zheevd": dw}
+            out["variants"].append(row)
+            print(f"{meth:8s} n={n:5d} batch={b:3d} {ms:10.2f} ms {ms / b:9.3f} ms/matrix"
+                  f"  info!=0: {bad}" + ("" if dw is None else f"  max|dW|/max|W| vs zheevd {dw:.1e}"),
+                  flush=True)
+    G = gram(n2)
+    for b in (1, 4, 8, 16, 32, 64):
+        ms, bad, _ = timed("zheevd", n2, G, b, 2)
+        out["batch_scan"].append({"n": n2, "batch": b, "ms": ms, "ms_per_matrix": ms / b})
+        print(f"zheevd   n={n2:5d} batch={b:3d} {ms:10.2f} ms {ms / b:9.3f} ms/matrix", flush=True)
+    total = 64
+    A = torch.empty((total, n2, n2), dtype=torch.complex128, device=dev)
+    W = torch.empty((total, n2), dtype=torch.float64, device=dev)
+    E = torch.empty((total, n2), dtype=torch.float64, device=dev)
+    info = torch.zeros(total, dtype=torch.int32, device=dev)
+    for c in (1, 2, 4, 8):
+        per = total // c
+        streams = [torch.cuda.Stream(dev) for _ in range(c)]
+        hs = [handle(s) for s in streams]
+        best = float("inf")
+        for rep in range(3):
+            A.copy_(G.expand(total, n2, n2))
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+
+            def run(j):
+                sl = slice(j * per, (j + 1) * per)
+                call(hs[j], "zheevd", n2, A[sl], per, W[sl], E[sl], info[sl], None)
+
+            th = [threading.Thread(target=run, args=(j,)) for j in range(c)]
+            for t in th:
+                t.start()
+            for t in th:
+                t.join()
+            torch.cuda.synchronize()
+            if rep:
+                best = min(best, 1e3 * (time.perf_counter() - t0))
+        for h in hs:
+            rb.rocblas_destroy_handle(h)
+        out["concurrent"].append({"n": n2, "matrices": total, "calls": c, "ms": best,
+                                  "ms_per_matrix": best / total})
+        print(f"zheevd   n={n2:5d} {total} matrices as {c} concurrent calls of {per:2d}"
+              f" {best:10.2f} ms {best / total:9.3f} ms/matrix", flush=True)
+    print(json.dumps({"eig_probe": out}), flush=True)
+
+
+def ms_slow(meth, n):
+    """One timed repetition for the calls that take seconds."""
+    return meth in ("zheevdj", "zheevdx") and n > 512 or meth == "zheevj" and n >= 512
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -625,6 +789,9 @@ def main():
             K.grouped_gemm_descs(pd, torch.complex128, "cuda")
         torch.cuda.synchronize()
         print(json.dumps({"gemm_probe": [B, m, k], "launches": [3, 3]}), flush=True)
+        return
+    if a.eig_probe:
+        eig_probe(torch, a)
         return
     if a.copy_probe:
         src = torch.ones(a.copy_probe << 18, dtype=torch.float32, device="cuda")
