@@ -238,3 +238,27 @@ def test_eigh_retry_ladder(monkeypatch):
     monkeypatch.setattr(eig, "_eigh_checked", lambda A: (_ for _ in ()).throw(RuntimeError("x")))
     with pytest.raises(RuntimeError):
         eig.eigh_desc(G)
+
+
+def test_gc_paused_restores_the_collector_state():
+    """The batched engine pauses the cyclic collector for its circuit loops
+    and restores the caller's setting, also when the loop raises."""
+    import gc
+    from atlas_q.batched import _gc_paused
+    assert gc.isenabled()
+    with _gc_paused():
+        assert not gc.isenabled()
+    assert gc.isenabled()
+    try:
+        with _gc_paused():
+            raise ValueError
+    except ValueError:
+        pass
+    assert gc.isenabled()
+    gc.disable()
+    try:
+        with _gc_paused():
+            pass
+        assert not gc.isenabled()
+    finally:
+        gc.enable()
