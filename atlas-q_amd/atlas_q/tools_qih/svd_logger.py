@@ -1,51 +1,79 @@
-"""SVD-event log of the reference (tools_qih/svd_logger.py:14-41).
+"""SVD-event log in the reference's on-disk format (tools_qih/svd_logger.py:14-41).
 
-With ``QIH_SVD_LOG_DIR`` set, ``get_logger()`` returns a logger appending one
-JSON object per line to ``svd_events_<YYYYmmdd_HHMMSS>_<pid>.jsonl.gz`` in that
-directory, each record carrying ``host``, ``pid`` and ``run_ts`` plus the
-caller's fields (``tn_core.mps_apply_2q`` logs ``kind="svd_spectrum"`` events
-with ``S_top128`` and ``adaptive_keep``); otherwise a logger that drops
-everything.  The file format is the reference's, so logs written here feed
-its dataset tools and logs written there feed ``ft_dataset`` here.
+Format (the interface, kept so logs interoperate in both directions): with the
+environment variable ``QIH_SVD_LOG_DIR`` naming a directory, events go to
+``<dir>/svd_events_<YYYYmmdd_HHMMSS>_<pid>.jsonl.gz``, one compact JSON object
+per line, each carrying ``host``, ``pid`` and ``run_ts`` (the file's time
+stamp) before the caller's fields.  ``tn_core.mps_apply_2q`` logs
+``kind="svd_spectrum"`` events (``S_top128``, ``adaptive_keep``, ...).
+
+``EventLog`` is one object whatever the setting: without a directory it is
+inactive and ``log`` returns at once.  The file is created on the first event
+(an idle process leaves no empty archive), every line is flushed so a crashed
+run keeps what it logged, and writers on several threads are serialised.
 """
 from __future__ import annotations
 
 import gzip
 import json
 import os
+import socket
 import threading
 import time
 from pathlib import Path
+from typing import Optional
+
+ENV_DIR = "QIH_SVD_LOG_DIR"
 
 
-class _NullLogger:
-    active = False
+class EventLog:
+    """Appends JSON-line events to a gzip archive (inactive without a directory)."""
 
-    def log(self, **kwargs):
-        pass
-
-
-class _JSONLGZLogger:
-    active = True
-
-    def __init__(self, out_dir: str):
-        self.dir = Path(out_dir)
-        self.dir.mkdir(parents=True, exist_ok=True)
-        run = time.strftime("%Y%m%d_%H%M%S")
+    def __init__(self, directory: Optional[str]):
+        self.directory = Path(directory) if directory else None
+        stamp = time.strftime("%Y%m%d_%H%M%S")
         pid = os.getpid()
-        self.path = self.dir / f"svd_events_{run}_{pid}.jsonl.gz"
-        self._fh = gzip.open(self.path, "at", encoding="utf-8")
-        self._lock = threading.Lock()
-        host = os.uname().nodename if hasattr(os, "uname") else "unknown"
-        self.meta = {"host": host, "pid": pid, "run_ts": run}
+        self.header = {"host": socket.gethostname(), "pid": pid, "run_ts": stamp}
+        self.path = (self.directory / f"svd_events_{stamp}_{pid}.jsonl.gz"
+                     if self.directory is not None else None)
+        self._stream = None
+        self._guard = threading.Lock()
 
-    def log(self, **kwargs):
-        line = json.dumps({**self.meta, **kwargs}, separators=(",", ":"))
-        with self._lock:
-            self._fh.write(line + "\n")
-            self._fh.flush()
+    @property
+    def active(self) -> bool:
+        return self.path is not None
+
+    def _open(self):
+        self.directory.mkdir(parents=True, exist_ok=True)
+        return gzip.open(self.path, mode="at", encoding="utf-8")
+
+    def log(self, **fields) -> None:
+        if self.path is None:
+            return
+        record = dict(self.header)
+        record.update(fields)
+        text = json.dumps(record, separators=(",", ":")) + "\n"
+        with self._guard:
+            if self._stream is None:
+                self._stream = self._open()
+            self._stream.write(text)
+            self._stream.flush()
+
+    def close(self) -> None:
+        """Finish the archive (further events reopen it in append mode)."""
+        with self._guard:
+            if self._stream is not None:
+                self._stream.close()
+                self._stream = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+        return False
 
 
-def get_logger():
-    out = os.getenv("QIH_SVD_LOG_DIR")
-    return _JSONLGZLogger(out) if out else _NullLogger()
+def get_logger() -> EventLog:
+    """The process's event log as configured by ``QIH_SVD_LOG_DIR``."""
+    return EventLog(os.environ.get(ENV_DIR) or None)

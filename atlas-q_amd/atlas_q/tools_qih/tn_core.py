@@ -20,8 +20,9 @@ update, the one its learned rank predictors plug into:
 * the sqrt(S) split into both cores in the ``aq_rebuild_sqrt`` kernel
   (tn_core.py:374-384).
 
-The gate builders and ``mps_init_plus`` / ``mps_apply_1q`` keep the
-reference's definitions (tn_core.py:62-183).
+The gate builders follow the reference's definitions (tn_core.py:61-167),
+written from their closed forms; ``mps_init_plus`` / ``mps_apply_1q`` keep
+the reference's state and convention (tn_core.py:170-183).
 """
 from __future__ import annotations
 
@@ -73,83 +74,97 @@ def svd_robust(T: torch.Tensor, driver: str = "gesvdj"):
 
 
 # ---------------------------------------------------------------- gates
+# The gate family of tn_core.py:61-167 (names, angle conventions and the
+# entangler's composition are the reference's interface).  Built here from
+# closed forms: a rotation about Pauli axis P is cos(t/2) I - i sin(t/2) P,
+# evaluated in float64 from the angle rounded to float32 (the reference
+# evaluates its angles in float32, so the two agree to float32 rounding).
+_PAULI = {
+    "x": np.array([[0, 1], [1, 0]], np.complex128),
+    "y": np.array([[0, -1j], [1j, 0]], np.complex128),
+    "z": np.array([[1, 0], [0, -1]], np.complex128),
+}
+_ZZ = np.kron(_PAULI["z"], _PAULI["z"])
+_CZ = np.diag([1.0, 1.0, 1.0, -1.0]).astype(np.complex128)
+_CNOT = np.eye(4, dtype=np.complex128)[[0, 1, 3, 2]]     # |10> <-> |11>
+
+
+def _half_angle(t):
+    h = 0.5 * float(np.float32(t))
+    return np.cos(h), np.sin(h)
+
+
+def _pauli_rotation(axis: str, t) -> np.ndarray:
+    c, s = _half_angle(t)
+    return c * np.eye(2, dtype=np.complex128) - 1j * s * _PAULI[axis]
+
+
+def _to_torch(M: np.ndarray, device, dtype) -> torch.Tensor:
+    return torch.as_tensor(M, dtype=torch.complex128).to(device=device, dtype=dtype)
+
+
 def Rx(t, device, dtype):
-    t = torch.as_tensor(t, device=device, dtype=torch.float32)
-    c, s = torch.cos(t / 2), torch.sin(t / 2)
-    M = torch.empty((2, 2), device=device, dtype=dtype)
-    M[0, 0] = c
-    M[0, 1] = -1j * s
-    M[1, 0] = -1j * s
-    M[1, 1] = c
-    return M
-
-
-def Rz(t, device, dtype):
-    t = torch.as_tensor(t, device=device, dtype=torch.float32)
-    c, s = torch.cos(t / 2), torch.sin(t / 2)
-    M = torch.zeros((2, 2), device=device, dtype=dtype)
-    M[0, 0] = (c - 1j * s).to(dtype)
-    M[1, 1] = (c + 1j * s).to(dtype)
-    return M
+    return _to_torch(_pauli_rotation("x", t), device, dtype)
 
 
 def Ry(t, device, dtype):
-    t = torch.as_tensor(t, device=device, dtype=torch.float32)
-    c, s = torch.cos(t / 2), torch.sin(t / 2)
-    M = torch.empty((2, 2), device=device, dtype=dtype)
-    M[0, 0] = c
-    M[0, 1] = -s
-    M[1, 0] = s
-    M[1, 1] = c
-    return M
+    return _to_torch(_pauli_rotation("y", t), device, dtype)
+
+
+def Rz(t, device, dtype):
+    return _to_torch(_pauli_rotation("z", t), device, dtype)
 
 
 def CZ_gate(device, dtype):
-    return torch.diag(torch.tensor([1, 1, 1, -1], device=device, dtype=dtype))
+    return _to_torch(_CZ, device, dtype)
 
 
 def CNOT_gate(device, dtype):
-    M = torch.zeros((4, 4), device=device, dtype=dtype)
-    M[0, 0] = M[1, 1] = M[2, 3] = M[3, 2] = 1
-    return M
+    return _to_torch(_CNOT, device, dtype)
 
 
 def RZZ_gate(theta, device, dtype):
-    """exp(-i theta/2 Z x Z)."""
-    t = torch.as_tensor(theta, device=device, dtype=torch.float32)
-    c, s = torch.cos(t / 2).to(dtype), torch.sin(t / 2).to(dtype)
-    return torch.diag(torch.stack([c - 1j * s, c + 1j * s, c + 1j * s, c - 1j * s]))
+    """exp(-i theta/2 Z x Z) = cos(theta/2) I - i sin(theta/2) Z x Z."""
+    return _to_torch(_zz_rotation(theta), device, dtype)
 
 
 def rand_su4_gate(device, dtype):
-    """QR of a complex Gaussian, determinant phase removed (tn_core.py:111-119)."""
-    A = torch.randn(4, 4, device=device) + 1j * torch.randn(4, 4, device=device)
-    Q, _ = torch.linalg.qr(A)
-    phase = torch.angle(torch.linalg.det(Q))
-    return (Q * torch.exp(-1j * phase / 4)).to(dtype)
+    """A random SU(4) gate drawn from torch's generator as the reference draws
+    it (tn_core.py:111-119: real then imaginary Gaussian parts, the Q factor
+    of their QR, determinant phase divided out), so a seed gives its gate."""
+    re = torch.randn(4, 4, device=device)
+    im = torch.randn(4, 4, device=device)
+    Q = torch.linalg.qr(torch.complex(re, im))[0]
+    phi = torch.angle(torch.linalg.det(Q))
+    return (Q * torch.polar(torch.ones_like(phi), -phi / 4)).to(dtype)
+
+
+def _euler_zxz(a, b, c) -> np.ndarray:
+    return _pauli_rotation("z", a) @ _pauli_rotation("x", b) @ _pauli_rotation("z", c)
+
+
+def _zz_rotation(theta) -> np.ndarray:
+    c, s = _half_angle(theta)
+    return c * np.eye(4, dtype=np.complex128) - 1j * s * _ZZ
+
+
+_ENTANGLING = {"cz": lambda th: _CZ, "cnot": lambda th: _CNOT, "rzz": _zz_rotation}
 
 
 def build_entangler(entangler: str, theta: float, device, dtype):
-    """U_right @ E @ U_left with Rz Rx Rz locals at angles theta*(k+1),
-    k = 0..11 (tn_core.py:123-167); "none" is the identity, "su4" random."""
+    """Two layers of Rz Rx Rz rotations on both qubits around an entangling
+    gate, rotation angles theta * (1 .. 12) in order (tn_core.py:123-167);
+    "none" gives the identity and "su4" a random gate."""
     if entangler == "none":
         return torch.eye(4, device=device, dtype=dtype)
     if entangler == "su4":
         return rand_su4_gate(device, dtype)
-    a, b, c, d, e, f, g, h, i, j, k, l = [theta * (n + 1) for n in range(12)]
-    U_left = torch.kron(Rz(a, device, dtype) @ Rx(b, device, dtype) @ Rz(c, device, dtype),
-                        Rz(d, device, dtype) @ Rx(e, device, dtype) @ Rz(f, device, dtype))
-    if entangler == "cz":
-        E = CZ_gate(device, dtype)
-    elif entangler == "cnot":
-        E = CNOT_gate(device, dtype)
-    elif entangler == "rzz":
-        E = RZZ_gate(theta, device, dtype)
-    else:
+    if entangler not in _ENTANGLING:
         raise ValueError(f"Unknown entangler: {entangler}")
-    U_right = torch.kron(Rz(g, device, dtype) @ Rx(h, device, dtype) @ Rz(i, device, dtype),
-                         Rz(j, device, dtype) @ Rx(k, device, dtype) @ Rz(l, device, dtype))
-    return U_right @ E @ U_left
+    ang = [theta * q for q in range(1, 13)]
+    before = np.kron(_euler_zxz(*ang[0:3]), _euler_zxz(*ang[3:6]))
+    after = np.kron(_euler_zxz(*ang[6:9]), _euler_zxz(*ang[9:12]))
+    return _to_torch(after @ _ENTANGLING[entangler](theta) @ before, device, dtype)
 
 
 # ---------------------------------------------------------------- MPS

@@ -228,10 +228,10 @@ def test_svd_events_match_oracle(tmp_path, monkeypatch):
                                                chi_max=16, adaptive=True, tol=1e-4)
             ocores, _, _ = T.mps_apply_2q(ocores, int(b), int(b) + 1, Ut, chi_max=16,
                                           adaptive=True, tol=1e-4, events=events)
-        tn_core.SVD_EVENT_LOGGER._fh.close()
-        rows, bad = ft_dataset.safe_load_events([tn_core.SVD_EVENT_LOGGER.path])
-        assert bad == 0 and len(rows) == len(events)
-        for (_, ev), oe in zip(rows, events):
+        tn_core.SVD_EVENT_LOGGER.close()
+        table = ft_dataset.read_events([tn_core.SVD_EVENT_LOGGER.path])
+        assert table.unreadable == 0 and len(table) == len(events)
+        for ev, oe in zip(table.events, events):
             assert ev["adaptive_keep"] == oe["adaptive_keep"]
             assert ev["pre_rank"] == oe["pre_rank"]
             assert np.allclose(ev["S_top128"], oe["S_top128"], rtol=1e-5, atol=1e-7)

@@ -238,13 +238,21 @@ def test_svd_logger_format_and_dataset_builder(tmp_path, monkeypatch):
                   S_top128=list(s) + [0.0] * 78, adaptive_keep=int(10 + f))
         log.log(**ev)
         evs.append(ev)
-    log._fh.close()
-    rows, bad = ft_dataset.safe_load_events([log.path])
-    assert bad == 0 and len(rows) == 10
-    assert {"host", "pid", "run_ts"} <= set(rows[0][1])
-    x, y = ft_dataset.sv_to_features_and_target(rows[3][1])
-    assert np.array_equal(x, T.features_ft135(np.array(evs[3]["S_top128"], np.float32)))
-    assert abs(float(y) - 13 / 50) < 1e-7
+    log.close()
+    table = ft_dataset.read_events([log.path])
+    assert table.unreadable == 0 and len(table) == 10
+    assert {"host", "pid", "run_ts"} <= set(table.events[0])
+    assert table.events[3]["S_top128"] == evs[3]["S_top128"]
+    X3 = ft_dataset.ft135_features(table.spectra[3])[0]
+    want = T.features_ft135(np.array(evs[3]["S_top128"], np.float32))
+    assert np.allclose(X3, want, rtol=1e-6, atol=1e-6)
+    y = ft_dataset.kept_fraction(table.spectra, table.keep)
+    assert abs(float(y[3]) - 13 / 50) < 1e-7
+    # all rows at once equal the per-event oracle
+    allx = ft_dataset.ft135_features(table.spectra)
+    for r, ev in enumerate(evs):
+        assert np.allclose(allx[r], T.features_ft135(np.array(ev["S_top128"], np.float32)),
+                           rtol=1e-6, atol=1e-6)
     X, Y, split, files, _ = ft_dataset.build_dataset([log.path])
     assert X.shape == (10, 135) and set(split.tolist()) <= {0, 1}
     monkeypatch.delenv("QIH_SVD_LOG_DIR")
