@@ -19,6 +19,7 @@ AQ_C64, AQ_C128 = 0, 1
 AQ_PREC_NATIVE, AQ_PREC_BF16X3 = 0, 1
 AQ_RANK_REF = 0
 AQ_RANK_TDVP = 1
+AQ_SAMPLE_SCRATCH_BYTES = 64 << 20
 
 c_i64 = ctypes.c_int64
 c_int = ctypes.c_int
@@ -85,11 +86,25 @@ SIGNATURES = {
     "aq_sample_workspace_bytes": (c_i64, [c_int, c_i64, c_vp, c_i64]),
     "aq_sample": (c_int, [c_int, c_int, c_int, c_i64, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp,
                           c_i64, c_vp]),
+    "aq_sample_noalloc": (c_int, [c_int, c_int, c_int, c_i64, c_vp, c_vp, c_i64, c_vp, c_vp,
+                                  c_vp, c_vp]),
     "aq_heev_small": (c_int, [c_i64, c_vp, c_i64, c_i64, c_vp, c_vp, c_i64, c_i64, c_i64, c_int,
                               ctypes.c_double, c_vp, c_vp]),
     "aq_heev_small_orders": (c_int, [c_i64, c_vp, c_vp, c_i64, c_i64, c_vp, c_vp, c_i64, c_i64,
                                      c_i64, c_int, ctypes.c_double, c_vp, c_vp]),
     "aq_heevd_batched": (c_int, [c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_vp, c_i64, c_vp]),
+    "aq_bgemm": (c_int, [c_int, c_i64, c_i64, c_i64, ctypes.c_double, c_vp, c_i64, c_i64, c_i64,
+                         c_int, c_vp, c_i64, c_i64, c_i64, c_int, ctypes.c_double, c_vp, c_i64,
+                         c_i64, c_i64, c_i64, c_vp]),
+    "aq_hetrd_batched": (c_int, [c_i64, c_vp, c_i64, c_vp, c_vp, c_vp, c_i64, c_vp]),
+    "aq_tridiag_eigvals": (c_int, [c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_i64, c_i64,
+                                   c_vp]),
+    "aq_tridiag_eigvecs_scratch_doubles": (c_i64, [c_i64, c_i64, c_i64]),
+    "aq_tridiag_eigvecs": (c_int, [c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64,
+                                   c_vp, c_i64, c_vp, c_i64, c_i64, c_vp]),
+    "aq_hetrd_vblocks": (c_int, [c_i64, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp]),
+    "aq_hetrd_tfactors": (c_int, [c_i64, c_i64, c_vp, c_i64, c_vp, c_vp, c_i64, c_vp]),
+    "aq_orth_band_reduce": (c_int, [c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp]),
     "aq_mlp_num_params": (c_int, []),
     "aq_mlp_sync_bf16": (c_int, [c_vp, c_vp, c_vp]),
     "aq_mlp_forward": (c_int, [c_vp, c_i64, c_vp, c_vp, c_vp, c_vp]),

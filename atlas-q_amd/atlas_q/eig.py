@@ -20,6 +20,8 @@ to its caller.
 """
 from __future__ import annotations
 
+import ctypes
+
 import torch
 
 JACOBI_MAX = 64
@@ -100,3 +102,105 @@ def eigh_desc(G: torch.Tensor):
         except RuntimeError as e:
             err = e
     raise err
+
+
+# ---------------------------------------------------------------- tridiagonal path
+NB_BACK = 64       # Householder vectors per block of the MFMA back-transformation
+ORTH_TILE = 64     # rows per diagonal tile of the orthogonality re-check
+ORTH_TOL = 1e-10   # largest |U_k^H U_k - I| entry accepted (else the matrix falls back)
+
+
+class GramTridiag:
+    """Leading eigenpairs of a batch of Hermitian complex128 Gram cores
+    [B, n, n] (row-major, lower triangle read, overwritten) through the
+    tridiagonal form: ``eigenvalues`` gives the spectra by bisection, and
+    ``vectors(k)`` the eigenvectors of the k[b] largest eigenvalues of matrix
+    b by twisted inverse iteration, back-transformed on MFMA, with the FP64
+    orthogonality re-check of the result."""
+
+    def __init__(self, Gb: torch.Tensor):
+        from . import kernels as K
+        from ._ffi import check, lib
+        if Gb.dtype != torch.complex128 or not Gb.is_contiguous() or Gb.dim() != 3:
+            raise ValueError("GramTridiag needs a contiguous complex128 batch [B, n, n]")
+        self.G = Gb
+        self.B, self.n = int(Gb.shape[0]), int(Gb.shape[1])
+        B, n, dev = self.B, self.n, Gb.device
+        self.D = torch.empty((B, n), dtype=torch.float64, device=dev)
+        self.E = torch.empty((B, max(n - 1, 1)), dtype=torch.float64, device=dev)
+        self.tau = torch.empty((B, max(n - 1, 1)), dtype=torch.complex128, device=dev)
+        check(lib().aq_hetrd_batched(n, K._ptr(Gb), n * n, K._ptr(self.D), K._ptr(self.E),
+                                     K._ptr(self.tau), B, K._stream(Gb)), "aq_hetrd_batched")
+        self.W = None
+
+    def eigenvalues(self) -> torch.Tensor:
+        """All eigenvalues, descending [B, n] (float64)."""
+        from . import kernels as K
+        from ._ffi import check, lib
+        B, n = self.B, self.n
+        self.W = torch.empty((B, n), dtype=torch.float64, device=self.G.device)
+        check(lib().aq_tridiag_eigvals(n, K._ptr(self.D), n, K._ptr(self.E), n - 1,
+                                       K._ptr(self.W), n, n, B, K._stream(self.G)),
+              "aq_tridiag_eigvals")
+        return self.W
+
+    def vectors(self, ks):
+        """(Et [B, kpad, n] complex128, err [B] float64 on the device): row j <
+        ks[b] of Et[b] is the conjugate transpose of the eigenvector of the
+        j-th largest eigenvalue (rows >= ks[b] are zero); err[b] the largest
+        |Et Et^H - I| entry over the diagonal tiles of the first ks[b] rows."""
+        from . import kernels as K
+        from ._ffi import check, lib
+        B, n, dev = self.B, self.n, self.G.device
+        st = K._stream(self.G)
+        L = lib()
+        kmax = max(1, max(int(k) for k in ks))
+        kpad = -(-kmax // ORTH_TILE) * ORTH_TILE
+        kv = torch.tensor([int(k) for k in ks], dtype=torch.int32).to(dev)
+        flat = torch.zeros(B * kpad * n + ORTH_TILE * n, dtype=torch.complex128, device=dev)
+        Et = flat[:B * kpad * n].view(B, kpad, n)
+        nsc = int(L.aq_tridiag_eigvecs_scratch_doubles(n, kpad, B))
+        sc = torch.empty(nsc, dtype=torch.float64, device=dev)
+        check(L.aq_tridiag_eigvecs(n, K._ptr(self.D), n, K._ptr(self.E), n - 1, K._ptr(self.W),
+                                   n, K._ptr(kv), kpad, K._ptr(Et), kpad * n, K._ptr(sc), nsc,
+                                   B, st), "aq_tridiag_eigvecs")
+        del sc
+        nb = NB_BACK
+        nblk = -(-(n - 1) // nb)
+        V = torch.empty((B, nblk, nb, n), dtype=torch.complex128, device=dev)
+        check(L.aq_hetrd_vblocks(n, nb, K._ptr(self.G), n * n, K._ptr(V), B, st),
+              "aq_hetrd_vblocks")
+        Gv = torch.empty((B, nblk, nb, nb), dtype=torch.complex128, device=dev)
+        bg = L.aq_bgemm
+        check(bg(1, nb, nb, n, 1.0, K._ptr(V), n, 1, nb * n, 1, K._ptr(V), 1, n, nb * n, 0, 0.0,
+                 K._ptr(Gv), nb, 1, nb * nb, B * nblk, st), "aq_bgemm (block Gram)")
+        Tf = torch.empty_like(Gv)
+        check(L.aq_hetrd_tfactors(n, nb, K._ptr(self.tau), n - 1, K._ptr(Gv), K._ptr(Tf), B, st),
+              "aq_hetrd_tfactors")
+        Vt2 = torch.empty_like(V)
+        check(bg(1, nb, n, nb, 1.0, K._ptr(Tf), nb, 1, nb * nb, 1, K._ptr(V), n, 1, nb * n, 0,
+                 0.0, K._ptr(Vt2), n, 1, nb * n, B * nblk, st), "aq_bgemm (conj(T') V)")
+        Wt = torch.empty((B, nb, kpad), dtype=torch.complex128, device=dev)
+        vb, eb = nblk * nb * n, kpad * n
+        for p in range(nblk):
+            Kp = min(p * nb + nb - 1, n - 2) + 1            # rows the block's vectors touch
+            off = p * nb * n * 16                          # bytes into V / Vt2 of matrix 0
+            vp = ctypes.c_void_p(V.data_ptr() + off)
+            tp = ctypes.c_void_p(Vt2.data_ptr() + off)
+            # Wt = conj(V_p) Et^T   [nb, kpad]
+            check(bg(1, nb, kpad, Kp, 1.0, vp, n, 1, vb, 1, K._ptr(Et), 1, n, eb, 0, 0.0,
+                     K._ptr(Wt), kpad, 1, nb * kpad, B, st), "aq_bgemm (back 1)")
+            # Et[:, :, :Kp] -= Wt^T (conj(T') V_p)
+            check(bg(1, kpad, Kp, nb, -1.0, K._ptr(Wt), 1, kpad, nb * kpad, 0, tp, n, 1, vb, 0,
+                     1.0, K._ptr(Et), n, 1, eb, B, st), "aq_bgemm (back 2)")
+        del V, Vt2, Gv, Tf, Wt
+        ntile = kpad // ORTH_TILE
+        Q = torch.empty((B, ntile, ORTH_TILE, 2 * ORTH_TILE), dtype=torch.complex128, device=dev)
+        check(bg(1, ORTH_TILE, 2 * ORTH_TILE, n, 1.0, K._ptr(Et), n, 1, ORTH_TILE * n, 0,
+                 K._ptr(Et), 1, n, ORTH_TILE * n, 1, 0.0, K._ptr(Q), 2 * ORTH_TILE, 1,
+                 ORTH_TILE * 2 * ORTH_TILE, B * ntile, st), "aq_bgemm (orthogonality)")
+        err = torch.empty(B, dtype=torch.float64, device=dev)
+        check(L.aq_orth_band_reduce(K._ptr(Q), ntile, K._ptr(kv), kpad, K._ptr(err), B, st),
+              "aq_orth_band_reduce")
+        self._keep = (flat, kv, Q)
+        return Et, err

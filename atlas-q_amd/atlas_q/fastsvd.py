@@ -44,6 +44,7 @@ class FactorStats:
         self.predicted = []   # (predicted width, kept rank)
         self.rejects = {}     # why the randomized factorization fell back
         self.tails = []       # uncaptured weight / ||X||^2 per randomized attempt
+        self.orth = []        # FP64 orthogonality re-check of the tridiagonal path
         self.lock = threading.Lock()
 
     def bump(self, key):
@@ -129,6 +130,20 @@ def _finish_from_gram(mps, X, U, S, Vr, X64, k_dtype, cap, budget_ok, cl, cr, ep
     return A_new, B_new, k, eps_loc, entropy, condS, local
 
 
+def plain_factor(mps, X, cap, budget_ok, local, cl, cr):
+    """The plain method's factorization of theta X (adaptive_mps.py:249-290):
+    SVD cascade, rank rule, and the complex128 recompute when condS exceeds
+    the policy's promotion threshold.  Returns the update's result tuple."""
+    U, S, Vh, driver = robust_svd(X)
+    k, e, s, c = choose_rank_from_sigma(S, mps.eps_bond, cap, budget_ok)
+    if math.isfinite(c) and c > mps.dtype_policy.promote_if_cond_gt:
+        U, S, Vh, driver = robust_svd(X.to(torch.complex128))
+        k, e, s, c = choose_rank_from_sigma(S, mps.eps_bond, cap, budget_ok)
+        local = torch.complex128
+    A_new, B_new = K.rebuild(U, S, Vh, k, cl, cr)
+    return A_new, B_new, k, e, s, c, driver, local
+
+
 def two_site_update(mps, i: int, A: torch.Tensor, B: torch.Tensor, gate: np.ndarray, cap: int,
                     budget_ok: Callable[[int], bool], local: torch.dtype):
     """Returns (A_new, B_new, k, eps_local, entropy, condS, driver, dtype)."""
@@ -162,12 +177,17 @@ def prepare_update(mps, i: int, A: torch.Tensor, B: torch.Tensor, gate: np.ndarr
                   "bf16_mfma" if prec == "bf16x3" else "fp32_mfma"):
         K.theta(A, B, gate, prec=prec, out=X)
     if local == torch.complex128 or method == "plain_in_fast":
-        U, S, Vh, driver = robust_svd(X)
-        k, e, s, c = choose_rank_from_sigma(S, mps.eps_bond, cap, budget_ok)
-        A_new, B_new = K.rebuild(U, S, Vh, k, cl, cr)
         st.bump("plain")
-        return A_new, B_new, k, e, s, c, driver, local
+        return plain_factor(mps, X, cap, budget_ok, local, cl, cr)
     feats = None
+    if (method == "learned" and getattr(mps, "harvest", None) is None and cm >= cap
+            and 2 * cap >= min(X.shape)):
+        # the bond already sits at the cap and a randomized sketch as wide as
+        # the cap would cover half the spectrum or more: the exact Gram method
+        # serves the update whatever the network predicts, so the sketch and
+        # the prediction are not paid for
+        st.reject("at_cap")
+        return GramJob(X, cap, budget_ok, local, cl, cr, None)
     pred = getattr(mps, "rank_predictor", None)
     if pred is None and method == "learned":
         with _INIT_LOCK:
@@ -247,7 +267,10 @@ def _gram_group(mps, jobs, st):
     work = float(sum(4.0 * m * m * x.shape[1] for x in X64))
     with RL.timed("gram_fp64_herk_grouped", work, "fp64_mfma"):
         K.grouped_gemm_descs(gd, c128, dev, keep)
-    with RL.timed("eig_zheevd_batched", B * 16.0 * m ** 3 / 3.0, "hbm"):
+    # algorithmic flops of a complex Hermitian eigen-decomposition with all
+    # vectors: tridiagonalisation 16/3 n^3, tridiagonal solve ~4/3 n^3,
+    # back-transformation 8 n^3
+    with RL.timed("eig_zheevd_batched", B * (16.0 / 3.0 + 4.0 / 3.0 + 8.0) * m ** 3, "fp64_mfma"):
         lam, _, ok = heevd_batched(Gb, inplace=True, return_ok=True)
     S64 = torch.sqrt(torch.clamp(lam, min=0.0)).contiguous()          # [B, m] descending
     ranks = _rank_batched(S64.to(torch.float32).contiguous(), [j.cap for j in jobs],
@@ -314,6 +337,108 @@ def _gram_group(mps, jobs, st):
     return out
 
 
+EIG_SOLVER = "tridiag"   # "tridiag": the library's tridiagonal top-k solver; "heevd": rocSOLVER
+
+
+def _gram_group_tridiag(mps, jobs, st):
+    """``_gram_group`` with the eigen-decomposition through the tridiagonal
+    form: one batched tridiagonalisation, all eigenvalues by bisection, the
+    rank rule on them, then eigenvectors only for the kept ranks (twisted
+    inverse iteration, MFMA back-transformation, FP64 orthogonality
+    re-check); a matrix whose vectors fail the re-check takes the per-job
+    path.  The projection reads the back-transformed rows, which are the rows
+    of U_k^H."""
+    from ._ffi import AQ_GROUPED_HERK, COPY_DESC, check, lib
+    from .eig import ORTH_TOL, GramTridiag
+    c128 = torch.complex128
+    B = len(jobs)
+    m = jobs[0].X.shape[0]
+    dev = jobs[0].X.device
+    keep = []
+    X64 = [j.X.to(c128) for j in jobs]
+    Gb = torch.zeros((B, m, m), dtype=c128, device=dev)
+    gd = np.stack([K.gemm_desc(X64[b], X64[b].T, Gb[b], conj_b=True, kind=AQ_GROUPED_HERK)
+                   for b in range(B)])
+    work = float(sum(4.0 * m * m * x.shape[1] for x in X64))
+    with RL.timed("gram_fp64_herk_grouped", work, "fp64_mfma"):
+        K.grouped_gemm_descs(gd, c128, dev, keep)
+    # algorithmic flops: complex Hermitian tridiagonalisation 16/3 n^3; the
+    # Sturm counts of bisection ~53 steps of 4 flops per entry and eigenvalue
+    with RL.timed("eig_zhetrd_batched", B * 16.0 / 3.0 * m ** 3, "fp64_mfma"):
+        tri = GramTridiag(Gb)
+    with RL.timed("eig_bisection", B * 53.0 * 4.0 * m * m, "fp64_valu"):
+        lam = tri.eigenvalues()
+    S64 = torch.sqrt(torch.clamp(lam, min=0.0)).contiguous()          # [B, m] descending
+    ranks = _rank_batched(S64.to(torch.float32).contiguous(), [j.cap for j in jobs],
+                          mps.eps_bond)
+    out = [None] * B
+    good = []
+    for b, (j, r) in enumerate(zip(jobs, ranks)):
+        if r.flags & 1:
+            k, e, sv, c = 1, 0.0, 0.0, float("inf")
+        else:
+            k, e, sv, c = int(r.k), float(r.eps_local), float(r.entropy), float(r.condS)
+        if math.isfinite(c) and c > mps.dtype_policy.promote_if_cond_gt:
+            continue                                   # per-job path promotes
+        good.append((b, k, e, sv, c))
+    if not good:
+        return out
+    kv = [0] * B
+    for b, k, _, _, _ in good:
+        kv[b] = k
+    kmax = max(kv)
+    with RL.timed("eig_vectors_backtransform", B * 8.0 * m * m * kmax, "fp64_mfma"):
+        Et, err = tri.vectors(kv)
+    errs = err.cpu().tolist()
+    st.orth.extend(errs[g[0]] for g in good)
+    good = [g for g in good if errs[g[0]] <= ORTH_TOL]
+    if not good:
+        return out
+    inv = torch.where(S64 > 0, 1.0 / S64, torch.zeros_like(S64))
+    ks = np.array([g[1] for g in good], np.int64)
+    ns = np.array([X64[g[0]].shape[1] for g in good], np.int64)
+    off = np.zeros(len(good), np.int64)
+    off[1:] = np.cumsum((ks * ns)[:-1])
+    P = torch.empty(int((ks * ns).sum()), dtype=c128, device=dev)
+    pd = []
+    for (b, k, _, _, _), o, n in zip(good, off, ns):
+        Vh = P[o:o + k * n].view(k, n)
+        pd.append(K.gemm_desc(Et[b], X64[b], Vh, M=k, K=m, N=n,
+                              a_ptr=Et[b].data_ptr(), a_strides=(m, 1)))
+    work = float(sum(8.0 * k * m * n for k, n in zip(ks, ns)))
+    with RL.timed("proj_fp64_grouped", work, "fp64_mfma"):
+        K.grouped_gemm_descs(np.stack(pd), c128, dev, keep)
+    cps = np.zeros(2 * len(good), COPY_DESC)
+    res = []
+    for q, ((b, k, e, sv, c), o, n) in enumerate(zip(good, off, ns)):
+        j = jobs[b]
+        A_new = torch.empty((j.cl, 2, k), dtype=torch.complex64, device=dev)
+        B_new = torch.empty((k, 2, j.cr), dtype=torch.complex64, device=dev)
+        a, bb = cps[2 * q], cps[2 * q + 1]
+        # A'[r, c] = U[r, c] S[c], U[r, c] = conj(Et[c, r])
+        a["src"], a["dst"] = Et[b].data_ptr(), A_new.data_ptr()
+        a["col_scale"] = S64[b].data_ptr()
+        a["rows"], a["cols"], a["s0"], a["s1"], a["conj"] = m, k, 1, m, 1
+        # B'[r, c] = Vh[r, c] / S[r]
+        bb["src"], bb["dst"] = P[o:].data_ptr(), B_new.data_ptr()
+        bb["row_scale"] = inv[b].data_ptr()
+        bb["rows"], bb["cols"], bb["s0"], bb["s1"] = k, n, n, 1
+        res.append((b, (A_new, B_new, k, e, sv, c, "gram_fp64", j.local)))
+    cnt = cps["rows"] * cps["cols"]
+    pre = np.zeros(len(cnt), np.int64)
+    pre[1:] = np.cumsum(cnt[:-1])
+    dd = torch.from_numpy(cps.view(np.uint8).copy()).to(dev)
+    pp = torch.from_numpy(pre).to(dev)
+    total = int(cnt.sum())
+    check(lib().aq_grouped_copy(K._ptr(dd), K._ptr(pp), len(cps), total,
+                                int(min(-(-total // 256), 65536)), K._stream(Gb)),
+          "aq_grouped_copy")
+    keep += [dd, pp, P, inv, S64, Gb, X64, Et, tri]
+    for b, r in res:
+        out[b] = r
+    return out
+
+
 def finish_gram(mps, jobs):
     """Exact Gram factorization of several updates.  Gram matrices of equal
     order are factored together: orders <= 64 in the batched Jacobi kernel,
@@ -334,7 +459,8 @@ def finish_gram(mps, jobs):
                    and jobs[i].X.dtype == torch.complex64]
         if (n > JACOBI_MAX and len(grouped) > 1 and mps.budget_global_mb is None
                 and GROUPED_GRAM):
-            res = _gram_group(mps, [jobs[i] for i in grouped], st)
+            group = _gram_group_tridiag if EIG_SOLVER == "tridiag" else _gram_group
+            res = group(mps, [jobs[i] for i in grouped], st)
             for i, r in zip(grouped, res):
                 if r is not None:
                     done[i] = r
@@ -355,7 +481,8 @@ def finish_gram(mps, jobs):
             grams[idx] = (None, left, X64)       # Gb[b] is overwritten below
         # streamed bytes of the one-stage tridiagonalisation: the trailing
         # n-j x n-j block of every matrix is read once per column j
-        with RL.timed("eig_zheevd_batched", len(idxs) * 16.0 * n ** 3 / 3.0, "hbm"):
+        with RL.timed("eig_zheevd_batched", len(idxs) * (16.0 / 3.0 + 4.0 / 3.0 + 8.0) * n ** 3,
+                      "fp64_mfma"):
             lam, V, ok = heevd_batched(Gb, inplace=True, return_ok=True)
         for b, idx in enumerate(idxs):
             if ok[b]:
@@ -375,12 +502,10 @@ def finish_gram(mps, jobs):
                 eig[idx] = eigh_desc(G)
             except RuntimeError:
                 # the FP64 eigensolver failed twice: the plain method's SVD cascade
-                U, S, Vh, driver = robust_svd(j.X)
-                k, e, s, c = choose_rank_from_sigma(S, mps.eps_bond, j.cap, j.budget_ok)
-                A_new, B_new = K.rebuild(U, S, Vh, k, j.cl, j.cr)
+                res = plain_factor(mps, j.X, j.cap, j.budget_ok, j.local, j.cl, j.cr)
                 st.bump("plain")
-                outs.append((A_new, B_new, k, e, s, c, driver, j.local))
-                _harvest(mps, j.feats, k)
+                outs.append(res)
+                _harvest(mps, j.feats, res[2])
                 continue
         lam, V = eig[idx]
         S = torch.sqrt(torch.clamp(lam, min=0.0))

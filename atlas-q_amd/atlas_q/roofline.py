@@ -13,8 +13,10 @@ from collections import defaultdict
 
 import torch
 
-PEAK = {"fp64_mfma": 78.6, "fp32_mfma": 157.3, "bf16_mfma": 2500.0, "hbm": 8000.0}
-UNIT = {"fp64_mfma": "TFLOP/s", "fp32_mfma": "TFLOP/s", "bf16_mfma": "TFLOP/s", "hbm": "GB/s"}
+PEAK = {"fp64_mfma": 78.6, "fp64_valu": 78.6, "fp32_mfma": 157.3, "bf16_mfma": 2500.0,
+        "hbm": 8000.0}
+UNIT = {"fp64_mfma": "TFLOP/s", "fp64_valu": "TFLOP/s", "fp32_mfma": "TFLOP/s",
+        "bf16_mfma": "TFLOP/s", "hbm": "GB/s"}
 
 _enabled = False
 _records = []
@@ -43,7 +45,14 @@ def timed(name: str, work: float, bound: str, nbytes: float = 0.0):
     _records.append((name, s, e, float(work), bound, float(nbytes)))
 
 
-def summary():
+def summary(window_s: float = None, units: int = None):
+    """Per-kernel figures of the recorded launches.  The roofline object names
+    the kernel (or library call) with the largest total time, whoever wrote
+    it.  With the timed region's wall time ``window_s`` (and the number of
+    updates ``units`` it holds) it also reports the headline efficiency
+    sum_k max(F_k / P_k, B_k / BW) / window: the time every recorded launch
+    would take at its bound's peak on its algorithmic work, over the time the
+    region took."""
     if not _records:
         return None
     torch.cuda.synchronize()
@@ -56,23 +65,32 @@ def summary():
         a[3] = bound
         a[4] += nb
     kernels = {}
+    ideal_s = 0.0
     for name, (cnt, ms, work, bound, nb) in agg.items():
         mean_ms = ms / cnt
         per = work / cnt
         if bound == "hbm":
             achieved = per / (mean_ms * 1e-3) / 1e9
+            ideal = work / (PEAK["hbm"] * 1e9)
         else:
             achieved = per / (mean_ms * 1e-3) / 1e12
+            ideal = max(work / (PEAK[bound] * 1e12), nb / (PEAK["hbm"] * 1e9))
+        ideal_s += ideal
         kernels[name] = {"launches": cnt, "mean_ms": mean_ms, "work_per_launch": per,
                          "bound": bound, "achieved": achieved, "peak": PEAK[bound],
                          "unit": UNIT[bound], "frac": achieved / PEAK[bound],
-                         "total_ms": ms}
-    # the dominant single kernel; library calls made of many kernels (names
-    # starting with "eig_") are reported in "all" only
-    single = [k for k in kernels if not k.startswith("eig_")] or list(kernels)
-    dom = max(single, key=lambda k: kernels[k]["total_ms"])
+                         "total_ms": ms, "ideal_ms": ideal * 1e3}
+    dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
     d = kernels[dom]
-    return {"kernel": dom, "bound": "mfma" if d["bound"] != "hbm" else "hbm",
-            "achieved": d["achieved"], "peak": d["peak"], "unit": d["unit"],
-            "frac": d["frac"], "traffic": None, "mean_launch_ms": d["mean_ms"],
-            "all": kernels}
+    out = {"kernel": dom, "bound": "hbm" if d["bound"] == "hbm" else "mfma",
+           "achieved": d["achieved"], "peak": d["peak"], "unit": d["unit"],
+           "frac": d["frac"], "traffic": None, "mean_launch_ms": d["mean_ms"],
+           "all": kernels}
+    if window_s:
+        out["headline_efficiency"] = {
+            "value": ideal_s / window_s,
+            "what": "sum over the recorded launches of max(algorithmic flops / peak, "
+                    "algorithmic bytes / 8 TB/s), divided by the timed region's wall time",
+            "ideal_ms_per_update": ideal_s * 1e3 / units if units else None,
+            "window_ms_per_update": window_s * 1e3 / units if units else None}
+    return out

@@ -11,6 +11,7 @@
 //      reference's probabilities (|.|^2 sums, +1e-15 floor, normalise, in the
 //      tensor's real type), draw outcome = (u >= p0), set the bit and the
 //      gather offset for the next site.
+#include <mutex>
 #include <type_traits>
 
 #include "bf16x3.h"
@@ -210,4 +211,57 @@ extern "C" int aq_sample(int dtype, int prec, int flags, int64_t n, const void* 
   AQ_REQUIRE(prec == AQ_PREC_NATIVE, AQ_ERR_ARG, "aq_sample: complex128 is FP64 only");
   return sample_impl<double>(prec, flags, n, sites, chi, shots, uniforms, bits, p0_out, workspace,
                              workspace_bytes, stream);
+}
+
+// ------------------------------------------------------------ no-allocation
+// aq_sample_noalloc: the sampler with its workspace in a file-scope device
+// array instead of caller memory.  Each GPU holds its own copy of `scratch`
+// for the life of the process; in host code the name is only a stand-in, so
+// the array is reached through the device address hipGetSymbolAddress gives
+// for the current device (looked up once per device, kept in scratch_ptr).
+// Nothing in the workspace has to start at zero: every site's T rows and
+// |T|^2 partials are written by the GEMM before the select reads them, and
+// the gather offsets are written by a site's select before the next site's
+// GEMM reads them (the first site reads none).
+namespace aq {
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) char scratch[AQ_SAMPLE_SCRATCH_BYTES];
+static void* scratch_ptr[kMaxDevices];
+static std::mutex scratch_lock;
+
+static int scratch_address(void** out) {
+  int dev = 0;
+  AQ_HIP(hipGetDevice(&dev));
+  AQ_REQUIRE(dev >= 0 && dev < kMaxDevices, AQ_ERR_UNSUPPORTED,
+             "aq_sample_noalloc: device index %d beyond %d", dev, kMaxDevices);
+  std::lock_guard<std::mutex> g(scratch_lock);
+  if (scratch_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    AQ_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    AQ_REQUIRE(p != nullptr, AQ_ERR_HIP, "aq_sample_noalloc: no device address for scratch");
+    scratch_ptr[dev] = p;
+  }
+  *out = scratch_ptr[dev];
+  return AQ_OK;
+}
+}  // namespace aq
+
+extern "C" int aq_sample_noalloc(int dtype, int prec, int flags, int64_t n,
+                                 const void* const* sites, const int64_t* chi, int64_t shots,
+                                 const double* uniforms, uint64_t* bits, double* p0_out,
+                                 hipStream_t stream) {
+  AQ_REQUIRE(chi && n > 0 && shots > 0, AQ_ERR_ARG,
+             "aq_sample_noalloc: n, shots and chi are required");
+  AQ_REQUIRE(dtype == AQ_C64 || dtype == AQ_C128, AQ_ERR_ARG,
+             "aq_sample_noalloc: unknown dtype %d", dtype);
+  const int64_t need = aq_sample_workspace_bytes(dtype, n, chi, shots);
+  AQ_REQUIRE(need > 0 && need <= (int64_t)AQ_SAMPLE_SCRATCH_BYTES, AQ_ERR_ARG,
+             "aq_sample_noalloc: this sample needs %lld workspace bytes, above the fixed "
+             "%lld-byte scratch (fewer shots per call, or aq_sample with a caller workspace)",
+             (long long)need, (long long)AQ_SAMPLE_SCRATCH_BYTES);
+  void* ws = nullptr;
+  const int st = scratch_address(&ws);
+  if (st != AQ_OK) return st;
+  return aq_sample(dtype, prec, flags, n, sites, chi, shots, uniforms, bits, p0_out, ws,
+                   (int64_t)AQ_SAMPLE_SCRATCH_BYTES, stream);
 }

@@ -1,0 +1,465 @@
+// Top-k eigenpairs of batched Hermitian Gram cores from their tridiagonal
+// form (FP64):
+//
+//   aq_hetrd_batched       dense -> tridiagonal (rocSOLVER zhetrd, strided
+//                          batched; Householder vectors left in the matrix)
+//   aq_tridiag_eigvals     all or the leading eigenvalues by bisection on
+//                          Sturm counts, one thread per eigenvalue
+//   aq_tridiag_eigvecs     eigenvectors of the leading eigenvalues by one
+//                          step of twisted inverse iteration (twisted LDL^T
+//                          factorisation at the computed eigenvalue, twist
+//                          index at the smallest |gamma|), one thread each;
+//                          the tridiagonal splits where an off-diagonal is
+//                          negligible and copies of a repeated eigenvalue
+//                          are taken from different blocks
+//   aq_hetrd_vblocks       the Householder vectors regrouped into blocks of
+//                          nb for the MFMA back-transformation, with the
+//                          blocks' compact-WY factors
+//   aq_orth_band_check     the FP64 orthogonality re-check of the computed
+//                          vectors (Gram blocks along the diagonal)
+//
+// Layout.  The Gram matrices are row-major n x n.  rocSOLVER reads them as
+// column-major, i.e. as conj(G); with fill mode "upper" it works on our lower
+// triangle (the one the Gram launch computes) and leaves H_i's vector v_i
+// (v_i[i] = 1, v_i[r] = 0 for r > i) in row i + 1, columns 0 .. i-1, with
+// conj(G) = Q T Q^H, Q = H_{n-2} ... H_1 H_0, T real symmetric tridiagonal
+// (D, E).  An eigenvector z of T gives the eigenvector conj(Q z) of G.  The
+// eigenvectors are kept as rows: Zt[j, :] = z_j^T, and the back-transform
+// forms Et = Zt Q^T, whose row j is (Q z_j)^T = the conjugate transpose of
+// G's j-th eigenvector -- the row of U_k^H that the projection U_k^H X reads.
+#include <float.h>
+#include <math.h>
+
+#include <algorithm>
+
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include "aq_common.h"
+
+namespace aq {
+
+constexpr int kTriThreads = 256;
+constexpr int kTriMaxN = 4096;  // d, e^2 and e of one matrix in LDS (96 KB at n = 4096)
+
+__device__ __forceinline__ double guard_pivot(double q, double pivmin) {
+  return fabs(q) < pivmin ? -pivmin : q;
+}
+
+// Number of eigenvalues of the tridiagonal (d, e2 = e^2) block [s, t] below x.
+__device__ __forceinline__ int sturm_count(const double* d, const double* e2, int s, int t,
+                                           double x, double pivmin) {
+  double q = guard_pivot(d[s] - x, pivmin);
+  int c = q < 0.0;
+  for (int i = s + 1; i <= t; ++i) {
+    q = guard_pivot(d[i] - x - e2[i - 1] / q, pivmin);
+    c += q < 0.0;
+  }
+  return c;
+}
+
+struct TriBounds {
+  double gl, gu, pivmin, tnorm;
+};
+
+// Loads d and e of one matrix into LDS (e2 = e^2 with split entries zeroed
+// when `split`), and reduces the Gershgorin bounds over the work-group.
+__device__ TriBounds load_tridiag(const double* __restrict__ D, const double* __restrict__ E,
+                                  int n, double* d, double* e2, double* e, bool split) {
+  __shared__ double red[3][kTriThreads / 64];
+  double lo = INFINITY, hi = -INFINITY, emax = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const double di = D[i];
+    const double em = i > 0 ? fabs(E[i - 1]) : 0.0;
+    const double ep = i + 1 < n ? fabs(E[i]) : 0.0;
+    d[i] = di;
+    if (i + 1 < n) {
+      double ei = E[i];
+      if (split) {
+        // negligible coupling (the bisection criterion of LAPACK's dstebz):
+        // e_i^2 <= (eps^2 |d_i d_{i+1}|) + tiny
+        const double dn = D[i + 1];
+        if (ei * ei <= DBL_EPSILON * DBL_EPSILON * fabs(di * dn) + DBL_MIN) ei = 0.0;
+      }
+      if (e) e[i] = ei;
+      e2[i] = ei * ei;
+      emax = fmax(emax, ei * ei);
+    }
+    lo = fmin(lo, di - em - ep);
+    hi = fmax(hi, di + em + ep);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    lo = fmin(lo, __shfl_xor(lo, off));
+    hi = fmax(hi, __shfl_xor(hi, off));
+    emax = fmax(emax, __shfl_xor(emax, off));
+  }
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[0][w] = lo;
+    red[1][w] = hi;
+    red[2][w] = emax;
+  }
+  __syncthreads();
+  TriBounds b;
+  b.gl = red[0][0];
+  b.gu = red[1][0];
+  double em2 = red[2][0];
+  for (int q = 1; q < (int)(blockDim.x >> 6); ++q) {
+    b.gl = fmin(b.gl, red[0][q]);
+    b.gu = fmax(b.gu, red[1][q]);
+    em2 = fmax(em2, red[2][q]);
+  }
+  b.tnorm = fmax(fabs(b.gl), fabs(b.gu));
+  b.pivmin = DBL_MIN * fmax(1.0, em2);
+  return b;
+}
+
+// ---------------------------------------------------------------- bisection
+// W[b, j] = j-th largest eigenvalue, j < nval.
+template <int NMAX>
+__global__ __launch_bounds__(kTriThreads) void tri_bisect_kernel(
+    int n, const double* __restrict__ D, int64_t sD, const double* __restrict__ E, int64_t sE,
+    double* __restrict__ W, int64_t sW, int nval) {
+  __shared__ double d[NMAX], e2[NMAX];
+  const int64_t b = blockIdx.y;
+  const TriBounds tb = load_tridiag(D + b * sD, E + b * sE, n, d, e2, nullptr, false);
+  __syncthreads();
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nval) return;
+  const int a = n - 1 - j;  // ascending index of the eigenvalue
+  const double width = tb.gu - tb.gl;
+  double lo = tb.gl - 2.0 * DBL_EPSILON * tb.tnorm - 4.0 * tb.pivmin;
+  double hi = tb.gu + 2.0 * DBL_EPSILON * tb.tnorm + 4.0 * tb.pivmin;
+  const double atol = 2.0 * DBL_EPSILON * tb.tnorm + 4.0 * tb.pivmin;
+  // invariant: count(lo) <= a < count(hi)
+  for (int it = 0; it < 128 && hi - lo > atol; ++it) {
+    const double mid = 0.5 * (lo + hi);
+    if (mid <= lo || mid >= hi) break;
+    if (sturm_count(d, e2, 0, n - 1, mid, tb.pivmin) <= a) lo = mid;
+    else hi = mid;
+  }
+  (void)width;
+  W[b * sW + j] = 0.5 * (lo + hi);
+}
+
+// ---------------------------------------------------------------- inverse iteration
+// Zt[b, j, :] = unit eigenvector (real, stored as complex) of W[b, j] for
+// j < k[b]; rows k[b] .. kmax-1 are zeroed.  scratch: 3 * n * batch * kmax
+// doubles (L, U, D+ per thread, thread-minor so that steps coalesce).
+template <int NMAX>
+__global__ __launch_bounds__(kTriThreads) void tri_invit_kernel(
+    int n, const double* __restrict__ D, int64_t sD, const double* __restrict__ E, int64_t sE,
+    const double* __restrict__ W, int64_t sW, const int32_t* __restrict__ kvec, int kmax,
+    double2* __restrict__ Z, int64_t sZ, double* __restrict__ scratch, int64_t batch) {
+  __shared__ double d[NMAX], e2[NMAX], e[NMAX];
+  __shared__ int nsplit;
+  const int64_t b = blockIdx.y;
+  const TriBounds tb = load_tridiag(D + b * sD, E + b * sE, n, d, e2, e, true);
+  if (threadIdx.x == 0) nsplit = 0;
+  __syncthreads();
+  for (int i = threadIdx.x; i + 1 < n; i += blockDim.x)
+    if (e[i] == 0.0) atomicAdd(&nsplit, 1);
+  __syncthreads();
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= kmax) return;
+  double2* z = Z + b * sZ + (int64_t)j * n;
+  const int kb = kvec ? kvec[b] : kmax;
+  if (j >= kb) {
+    for (int i = 0; i < n; ++i) z[i] = make_double2(0.0, 0.0);
+    return;
+  }
+  const double* Wb = W + b * sW;
+  const double lam = Wb[j];
+  const double delta = 8.0 * DBL_EPSILON * tb.tnorm + 8.0 * tb.pivmin;
+  // block of the tridiagonal this vector lives in
+  int s = 0, t = n - 1;
+  if (nsplit > 0) {
+    int j0 = j;
+    while (j0 > 0 && Wb[j0 - 1] - lam <= delta) --j0;
+    const int copy = j - j0;
+    int acc = 0, bs = 0;
+    bool found = false;
+    for (int i = 0; i < n && !found; ++i) {
+      if (i == n - 1 || e[i] == 0.0) {  // block [bs, i]
+        const int c = sturm_count(d, e2, bs, i, lam + delta, tb.pivmin) -
+                      sturm_count(d, e2, bs, i, lam - delta, tb.pivmin);
+        if (acc + c > copy) {
+          s = bs;
+          t = i;
+          found = true;
+        }
+        acc += c;
+        bs = i + 1;
+      }
+    }
+  }
+  const int64_t T = batch * (int64_t)kmax;
+  const int64_t tid = b * kmax + j;
+  double* Ls = scratch;
+  double* Us = scratch + (int64_t)n * T;
+  double* Ps = scratch + 2 * (int64_t)n * T;
+#define AQ_SC(arr, i) arr[(int64_t)(i) * T + tid]
+  const double piv = tb.pivmin;
+  // stationary factorisation T - lam = L+ D+ L+^T (top down)
+  double dp = guard_pivot(d[s] - lam, piv);
+  AQ_SC(Ps, s) = dp;
+  for (int i = s; i < t; ++i) {
+    const double l = e[i] / dp;
+    AQ_SC(Ls, i) = l;
+    dp = guard_pivot(d[i + 1] - lam - l * e[i], piv);
+    AQ_SC(Ps, i + 1) = dp;
+  }
+  // progressive factorisation T - lam = U- D- U-^T (bottom up) and the twist
+  double dm = guard_pivot(d[t] - lam, piv);
+  int r = t;
+  double best = fabs(AQ_SC(Ps, t));
+  for (int i = t - 1; i >= s; --i) {
+    const double u = e[i] / dm;
+    AQ_SC(Us, i) = u;
+    dm = guard_pivot(d[i] - lam - u * e[i], piv);
+    const double g = fabs(AQ_SC(Ps, i) + dm - (d[i] - lam));
+    if (g < best) {
+      best = g;
+      r = i;
+    }
+  }
+  // z_r = 1, z_i = -L_i z_{i+1} above, z_{i+1} = -U_i z_i below
+  double nrm2 = 1.0, zi = 1.0;
+  for (int i = r - 1; i >= s; --i) {
+    zi = -AQ_SC(Ls, i) * zi;
+    nrm2 += zi * zi;
+  }
+  zi = 1.0;
+  for (int i = r; i < t; ++i) {
+    zi = -AQ_SC(Us, i) * zi;
+    nrm2 += zi * zi;
+  }
+  const double inv = 1.0 / sqrt(nrm2);
+  for (int i = 0; i < s; ++i) z[i] = make_double2(0.0, 0.0);
+  for (int i = t + 1; i < n; ++i) z[i] = make_double2(0.0, 0.0);
+  z[r] = make_double2(inv, 0.0);
+  zi = 1.0;
+  for (int i = r - 1; i >= s; --i) {
+    zi = -AQ_SC(Ls, i) * zi;
+    z[i] = make_double2(zi * inv, 0.0);
+  }
+  zi = 1.0;
+  for (int i = r; i < t; ++i) {
+    zi = -AQ_SC(Us, i) * zi;
+    z[i + 1] = make_double2(zi * inv, 0.0);
+  }
+#undef AQ_SC
+}
+
+// ---------------------------------------------------------------- back-transform blocks
+// Vblk[b, p, j, r] = v_i[r] (i = p*nb + j; zero row when i > n-2).
+__global__ __launch_bounds__(256) void hetrd_vblk_kernel(int n, int nb, int nblk,
+                                                         const double2* __restrict__ A,
+                                                         int64_t sA, double2* __restrict__ V) {
+  const int64_t b = blockIdx.z;
+  const int p = blockIdx.y;
+  const int64_t row0 = (int64_t)p * nb;
+  double2* Vp = V + (b * nblk + p) * (int64_t)nb * n;
+  const double2* Ab = A + b * sA;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < (int64_t)nb * n;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int j = (int)(idx / n), r = (int)(idx % n);
+    const int64_t i = row0 + j;
+    double2 v = make_double2(0.0, 0.0);
+    if (i <= n - 2) {
+      if (r < i) v = Ab[(i + 1) * n + r];
+      else if (r == i) v.x = 1.0;
+    }
+    Vp[idx] = v;
+  }
+}
+
+// Forward compact-WY factor of P^H = H_{i0}^H ... H_{i0+nb-1}^H from the
+// reflectors' Gram matrix Gv[a, c] = v_a^H v_c (nb x nb, row-major):
+// T'[j, j] = conj(tau_j), T'[0:j, j] = -conj(tau_j) T'[0:j, 0:j] Gv[0:j, j].
+// One work-group of nb threads per (matrix, block); Tf row-major nb x nb.
+__global__ void hetrd_tfactor_kernel(int n, int nb, int nblk, const double2* __restrict__ tau,
+                                     int64_t stau, const double2* __restrict__ Gv,
+                                     double2* __restrict__ Tf) {
+  __shared__ double2 t[64][65];
+  const int64_t b = blockIdx.y;
+  const int p = blockIdx.x;
+  const int a = threadIdx.x;
+  const int64_t off = (b * nblk + p) * (int64_t)nb * nb;
+  const double2* G = Gv + off;
+  for (int c = 0; c < nb; ++c) t[a][c] = make_double2(0.0, 0.0);
+  __syncthreads();
+  for (int j = 0; j < nb; ++j) {
+    const int64_t i = (int64_t)p * nb + j;
+    double2 tj = make_double2(0.0, 0.0);
+    if (i <= n - 2) {
+      tj = tau[b * stau + i];
+      tj.y = -tj.y;  // conj(tau)
+    }
+    // row a < j of column j: -conj(tau_j) sum_{c < j} T'[a, c] Gv[c, j]
+    if (a < j) {
+      double2 acc = make_double2(0.0, 0.0);
+      for (int c = a; c < j; ++c) acc = cadd(acc, cmul(t[a][c], G[(int64_t)c * nb + j]));
+      const double2 v = cmul(tj, acc);
+      t[a][j] = make_double2(-v.x, -v.y);
+    } else if (a == j) {
+      t[a][j] = tj;
+    }
+    __syncthreads();
+  }
+  for (int c = 0; c < nb; ++c) Tf[off + (int64_t)a * nb + c] = t[a][c];
+}
+
+// err[b] = max over a, c < k_b of |Q[b, tile, a, c] - delta_ac| where the
+// tiles are the diagonal Gram blocks Q[tile] = Zt[64 tile : 64 tile + 64]
+// Zt[64 tile : 64 tile + 128]^H (64 x 128, row-major) of each matrix.
+__global__ __launch_bounds__(256) void orth_band_reduce_kernel(const double2* __restrict__ Q,
+                                                               int ntile,
+                                                               const int32_t* __restrict__ kvec,
+                                                               int kmax, double* __restrict__ err) {
+  __shared__ double red[4];
+  const int64_t b = blockIdx.x;
+  const int kb = kvec ? kvec[b] : kmax;
+  double m = 0.0;
+  for (int64_t idx = threadIdx.x; idx < (int64_t)ntile * 64 * 128; idx += blockDim.x) {
+    const int tile = (int)(idx / (64 * 128));
+    const int rem = (int)(idx % (64 * 128));
+    const int a = tile * 64 + rem / 128, c = tile * 64 + rem % 128;
+    if (a >= kb || c >= kb) continue;
+    double2 q = Q[(b * ntile) * 64 * 128 + idx];
+    if (a == c) q.x -= 1.0;
+    m = fmax(m, sqrt(q.x * q.x + q.y * q.y));
+  }
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = red[0];
+    for (int q = 1; q < 4; ++q) r = fmax(r, red[q]);
+    err[b] = r;
+  }
+}
+
+namespace {
+struct Handle {
+  rocblas_handle h = nullptr;
+  ~Handle() {
+    if (h) rocblas_destroy_handle(h);
+  }
+};
+thread_local Handle t_handle;
+}  // namespace
+
+}  // namespace aq
+
+using namespace aq;
+
+extern "C" int aq_hetrd_batched(int64_t n, void* A, int64_t strideA, double* D, double* E,
+                                void* tau, int64_t batch, hipStream_t stream) {
+  AQ_REQUIRE(A && D && E && tau, AQ_ERR_ARG, "aq_hetrd_batched: null pointer");
+  AQ_REQUIRE(n >= 2 && n <= kTriMaxN, AQ_ERR_ARG, "aq_hetrd_batched: n out of range (%lld)",
+             (long long)n);
+  AQ_REQUIRE(batch >= 1 && batch <= 65535 && strideA >= n * n, AQ_ERR_ARG,
+             "aq_hetrd_batched: bad batch or stride");
+  if (!t_handle.h) {
+    rocblas_status s = rocblas_create_handle(&t_handle.h);
+    AQ_REQUIRE(s == rocblas_status_success, AQ_ERR_HIP,
+               "aq_hetrd_batched: rocblas_create_handle failed (%d)", (int)s);
+  }
+  rocblas_status s = rocblas_set_stream(t_handle.h, stream);
+  AQ_REQUIRE(s == rocblas_status_success, AQ_ERR_HIP, "aq_hetrd_batched: set_stream (%d)",
+             (int)s);
+  s = rocsolver_zhetrd_strided_batched(
+      t_handle.h, rocblas_fill_upper, (rocblas_int)n,
+      reinterpret_cast<rocblas_double_complex*>(A), (rocblas_int)n, (rocblas_stride)strideA, D,
+      (rocblas_stride)n, E, (rocblas_stride)(n - 1),
+      reinterpret_cast<rocblas_double_complex*>(tau), (rocblas_stride)(n - 1),
+      (rocblas_int)batch);
+  AQ_REQUIRE(s == rocblas_status_success, AQ_ERR_HIP,
+             "aq_hetrd_batched: rocsolver_zhetrd_strided_batched failed (%d)", (int)s);
+  return AQ_OK;
+}
+
+extern "C" int aq_tridiag_eigvals(int64_t n, const double* D, int64_t sD, const double* E,
+                                  int64_t sE, double* W, int64_t sW, int64_t nval, int64_t batch,
+                                  hipStream_t stream) {
+  AQ_REQUIRE(D && E && W, AQ_ERR_ARG, "aq_tridiag_eigvals: null pointer");
+  AQ_REQUIRE(n >= 1 && n <= kTriMaxN && nval >= 1 && nval <= n, AQ_ERR_ARG,
+             "aq_tridiag_eigvals: n %lld / nval %lld out of range", (long long)n,
+             (long long)nval);
+  AQ_REQUIRE(batch >= 1 && batch <= 65535, AQ_ERR_ARG, "aq_tridiag_eigvals: bad batch");
+  dim3 grid((unsigned)cdiv(nval, kTriThreads), (unsigned)batch);
+  if (n <= 2048)
+    tri_bisect_kernel<2048><<<grid, kTriThreads, 0, stream>>>((int)n, D, sD, E, sE, W, sW,
+                                                              (int)nval);
+  else
+    tri_bisect_kernel<kTriMaxN><<<grid, kTriThreads, 0, stream>>>((int)n, D, sD, E, sE, W, sW,
+                                                                  (int)nval);
+  AQ_LAUNCH_CHECK();
+  return AQ_OK;
+}
+
+extern "C" int64_t aq_tridiag_eigvecs_scratch_doubles(int64_t n, int64_t kmax, int64_t batch) {
+  return 3 * n * kmax * batch;
+}
+
+extern "C" int aq_tridiag_eigvecs(int64_t n, const double* D, int64_t sD, const double* E,
+                                  int64_t sE, const double* W, int64_t sW, const int32_t* kvec,
+                                  int64_t kmax, void* Z, int64_t sZ, double* scratch,
+                                  int64_t scratch_doubles, int64_t batch, hipStream_t stream) {
+  AQ_REQUIRE(D && E && W && Z && scratch, AQ_ERR_ARG, "aq_tridiag_eigvecs: null pointer");
+  AQ_REQUIRE(n >= 1 && n <= kTriMaxN && kmax >= 1 && kmax <= n, AQ_ERR_ARG,
+             "aq_tridiag_eigvecs: n %lld / kmax %lld out of range", (long long)n,
+             (long long)kmax);
+  AQ_REQUIRE(batch >= 1 && batch <= 65535 && sZ >= kmax * n, AQ_ERR_ARG,
+             "aq_tridiag_eigvecs: bad batch or stride");
+  AQ_REQUIRE(scratch_doubles >= 3 * n * kmax * batch, AQ_ERR_ARG,
+             "aq_tridiag_eigvecs: scratch too small");
+  dim3 grid((unsigned)cdiv(kmax, kTriThreads), (unsigned)batch);
+  if (n <= 2048)
+    tri_invit_kernel<2048><<<grid, kTriThreads, 0, stream>>>(
+        (int)n, D, sD, E, sE, W, sW, kvec, (int)kmax, (double2*)Z, sZ, scratch, batch);
+  else
+    tri_invit_kernel<kTriMaxN><<<grid, kTriThreads, 0, stream>>>(
+        (int)n, D, sD, E, sE, W, sW, kvec, (int)kmax, (double2*)Z, sZ, scratch, batch);
+  AQ_LAUNCH_CHECK();
+  return AQ_OK;
+}
+
+extern "C" int aq_hetrd_vblocks(int64_t n, int64_t nb, const void* A, int64_t strideA,
+                                void* V, int64_t batch, hipStream_t stream) {
+  AQ_REQUIRE(A && V, AQ_ERR_ARG, "aq_hetrd_vblocks: null pointer");
+  AQ_REQUIRE(n >= 2 && nb >= 1 && nb <= 64 && batch >= 1 && batch <= 65535, AQ_ERR_ARG,
+             "aq_hetrd_vblocks: bad sizes");
+  const int64_t nblk = cdiv(n - 1, nb);
+  dim3 grid((unsigned)std::min<int64_t>(cdiv(nb * n, 256), 64), (unsigned)nblk,
+            (unsigned)batch);
+  hetrd_vblk_kernel<<<grid, 256, 0, stream>>>((int)n, (int)nb, (int)nblk, (const double2*)A,
+                                               strideA, (double2*)V);
+  AQ_LAUNCH_CHECK();
+  return AQ_OK;
+}
+
+extern "C" int aq_hetrd_tfactors(int64_t n, int64_t nb, const void* tau, int64_t stau,
+                                 const void* Gv, void* Tf, int64_t batch, hipStream_t stream) {
+  AQ_REQUIRE(tau && Gv && Tf, AQ_ERR_ARG, "aq_hetrd_tfactors: null pointer");
+  AQ_REQUIRE(n >= 2 && nb >= 1 && nb <= 64 && batch >= 1 && batch <= 65535, AQ_ERR_ARG,
+             "aq_hetrd_tfactors: bad sizes");
+  const int64_t nblk = cdiv(n - 1, nb);
+  dim3 grid((unsigned)nblk, (unsigned)batch);
+  hetrd_tfactor_kernel<<<grid, (unsigned)nb, 0, stream>>>(
+      (int)n, (int)nb, (int)nblk, (const double2*)tau, stau, (const double2*)Gv, (double2*)Tf);
+  AQ_LAUNCH_CHECK();
+  return AQ_OK;
+}
+
+extern "C" int aq_orth_band_reduce(const void* Q, int64_t ntile, const int32_t* kvec,
+                                   int64_t kmax, double* err, int64_t batch,
+                                   hipStream_t stream) {
+  AQ_REQUIRE(Q && err, AQ_ERR_ARG, "aq_orth_band_reduce: null pointer");
+  AQ_REQUIRE(ntile >= 1 && batch >= 1, AQ_ERR_ARG, "aq_orth_band_reduce: bad sizes");
+  orth_band_reduce_kernel<<<(unsigned)batch, 256, 0, stream>>>(
+      (const double2*)Q, (int)ntile, kvec, (int)kmax, err);
+  AQ_LAUNCH_CHECK();
+  return AQ_OK;
+}

@@ -583,17 +583,21 @@ def marker(torch):
     torch.cuda.synchronize()
 
 
-def traffic_from_profile(kernel: str):
-    """HBM bytes per launch of the update path's grouped FP64 launch `kernel`
-    at the bench's shape, from the committed counter summary (FETCH_SIZE and
-    WRITE_SIZE runs of ``bench.py --gemm-probe``, calibrated on a copy of
-    known size); None if the summary has no figure for it."""
-    path = ROOT / "profiles" / "grouped_fp64_gemm_chi1024_traffic.md"
-    if not path.exists():
-        return None, None
-    for line in path.read_text().splitlines():
-        if line.startswith(f"traffic_bytes_per_launch[{kernel}]"):
-            return float(line.split("=")[1]), str(path.relative_to(ROOT))
+def traffic_from_profile(kernel: str, work_per_launch: float):
+    """HBM bytes per launch of `kernel` from the committed counter summaries
+    (FETCH_SIZE and WRITE_SIZE runs, calibrated on a copy of known size),
+    scaled from the profiled launch to this one by the ratio of their
+    algorithmic work (the profiled launch's work is recorded beside its
+    bytes); (None, None) when no summary covers the kernel."""
+    for path in sorted((ROOT / "profiles").glob("*traffic*.md")):
+        bytes_, work = None, None
+        for line in path.read_text().splitlines():
+            if line.startswith(f"traffic_bytes_per_launch[{kernel}]"):
+                bytes_ = float(line.split("=")[1])
+            elif line.startswith(f"work_per_launch[{kernel}]"):
+                work = float(line.split("=")[1])
+        if bytes_ is not None and work:
+            return bytes_ * (work_per_launch / work), str(path.relative_to(ROOT))
     return None, None
 
 
@@ -834,9 +838,11 @@ def main():
     RL.enable(False)
     ups = a.steps / t_fast
     log(f"fast path: {ups:.2f} updates/s")
-    roof = RL.summary()
-    if roof and chi == 1024 and a.sites == 128:
-        roof["traffic"], roof["traffic_source"] = traffic_from_profile(roof["kernel"])
+    roof = RL.summary(window_s=t_fast, units=a.steps)
+    if roof:
+        d = roof["all"][roof["kernel"]]
+        roof["traffic"], roof["traffic_source"] = traffic_from_profile(
+            roof["kernel"], d["work_per_launch"])
     if roof and a.workers > 1:
         # per-update launches go out from several worker streams at once: their
         # event pairs span the other streams' kernels as well
@@ -847,19 +853,18 @@ def main():
                                "'achieved' is a lower bound on this kernel's own rate")
     eig = None
     if roof:
-        eig = {}
-        for key, what in (("eig_zheevd_batched", "rocSOLVER strided-batched zheevd over the FP64 "
-                                                 "Gram cores of a sub-layer; bytes = n^3/3 "
-                                                 "complex128 per matrix"),):
-            if key in roof["all"]:
-                e = roof["all"][key]
-                eig[key] = {"what": what + " (the trailing blocks a one-stage "
-                                           "tridiagonalisation streams); HIP events around each call",
-                            "calls": e["launches"], "mean_ms_per_call": e["mean_ms"],
-                            "total_ms": e["total_ms"],
-                            "share_of_timed_region": e["total_ms"] / (1e3 * t_fast),
-                            "streamed_GBps": e["achieved"], "frac_of_hbm_peak": e["frac"]}
-        eig["ms_per_update"] = sum(v["total_ms"] for v in eig.values()) / a.steps
+        eig = {"calls": {}}
+        for key, e in roof["all"].items():
+            if key.startswith("eig_"):
+                eig["calls"][key] = {"calls": e["launches"], "mean_ms_per_call": e["mean_ms"],
+                                     "total_ms": e["total_ms"],
+                                     "share_of_timed_region": e["total_ms"] / (1e3 * t_fast),
+                                     "achieved": e["achieved"], "unit": e["unit"],
+                                     "frac": e["frac"]}
+        eig["ms_per_update"] = sum(v["total_ms"] for v in eig["calls"].values()) / a.steps
+        eig["solver"] = __import__("atlas_q.fastsvd", fromlist=["x"]).EIG_SOLVER
+        orth = getattr(getattr(fast, "factor_stats", None), "orth", [])
+        eig["max_orthogonality_error"] = max(orth) if orth else None
     ks = fast.statistics.logs["k_star"][a.warmup:]
     fast_seq = None
     if a.workers > 1 and a.seq_steps > 0:

@@ -143,6 +143,21 @@ int aq_sample(int dtype, int prec, int flags, int64_t n, const void* const* site
               int64_t shots, const double* uniforms, uint64_t* bits, double* p0_out,
               void* workspace, int64_t workspace_bytes, hipStream_t stream);
 
+/* aq_sample without a caller workspace (no reference counterpart: the
+ * reference's sampler allocates freely).  For callers that cannot allocate
+ * device memory -- code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc: this entry point calls no hipMalloc,
+ * hipFree or allocating library routine.  Its workspace is a fixed device
+ * array of AQ_SAMPLE_SCRATCH_BYTES per GPU, held until the process ends;
+ * a sample whose aq_sample_workspace_bytes(...) exceeds it is rejected with
+ * AQ_ERR_ARG.  Every call on one GPU shares that GPU's array: calls that
+ * overlap on one GPU (other streams, other host threads) race, so callers
+ * serialise them.  Results are aq_sample's, bit for bit. */
+#define AQ_SAMPLE_SCRATCH_BYTES (64ll << 20)
+int aq_sample_noalloc(int dtype, int prec, int flags, int64_t n, const void* const* sites,
+                      const int64_t* chi, int64_t shots, const double* uniforms, uint64_t* bits,
+                      double* p0_out, hipStream_t stream);
+
 /* Small Hermitian eigensolver (no reference counterpart: the reference calls
  * LAPACK/cuSOLVER SVD, linalg_robust.py:17-52).  Batched two-sided Jacobi in
  * FP64, one work-group per n x n matrix (n <= 64) held in LDS.  A: complex128,
@@ -174,6 +189,61 @@ int aq_heev_small_orders(int64_t n, const int64_t* orders, const void* A, int64_
  * info: int32 [batch] on the device, 0 on success (> 0: not converged). */
 int aq_heevd_batched(int64_t n, void* A, int64_t strideA, double* W, int64_t strideW, double* E,
                      int32_t* info, int64_t batch, hipStream_t stream);
+
+/* Strided-batched complex GEMM with a scaled accumulate (no reference
+ * counterpart: the steps of the Gram eigensolver below, which replaces the
+ * factorization of adaptive_mps.py:249-250 / linalg_robust.py:17-52 in the
+ * truncated-SVD methods):  C_b = alpha op(A_b) op(B_b) + beta C_b for
+ * b < batch, matrix b of each operand at its pointer + b * (its batch stride),
+ * element strides as aq_cgemm, real alpha / beta (beta == 0 does not read C). */
+int aq_bgemm(int dtype, int64_t M, int64_t N, int64_t K, double alpha, const void* A,
+             int64_t a_s0, int64_t a_s1, int64_t a_sb, int conj_a, const void* B, int64_t b_s0,
+             int64_t b_s1, int64_t b_sb, int conj_b, double beta, void* C, int64_t c_s0,
+             int64_t c_s1, int64_t c_sb, int64_t batch, hipStream_t stream);
+
+/* Gram eigensolver through the tridiagonal form, in FP64 (no reference
+ * counterpart: the reference factors theta with a thin SVD,
+ * adaptive_mps.py:249-250 via linalg_robust.py:17-52; the Gram method needs
+ * the leading eigenpairs of X X^H, and only those the rank rule keeps).
+ *
+ * aq_hetrd_batched: complex128 row-major Hermitian n x n matrices (lower
+ * triangle read) at A + b*strideA reduced to real tridiagonal form, D
+ * [batch, n], E [batch, n-1], tau complex128 [batch, n-1]; the Householder
+ * vectors stay in A: with Q = H_{n-2} ... H_0, H_i = I - tau_i v_i v_i^H,
+ * v_i[i] = 1, v_i[0:i] = row i+1, columns 0..i-1 of A, v_i zero below i,
+ * conj(G) = Q T Q^H.  (rocSOLVER zhetrd, strided batched.)
+ * aq_tridiag_eigvals: the nval largest eigenvalues of each tridiagonal,
+ * descending, into W (bisection on Sturm counts, absolute accuracy
+ * ~ 2 eps ||T||).
+ * aq_tridiag_eigvecs: for j < kvec[b] (all j < kmax when kvec is NULL), row
+ * j of Zt (complex128 [batch, >= kmax, n], matrix stride sZ) = the unit
+ * eigenvector of W[b, j] (twisted inverse iteration; rows kvec[b] .. kmax-1
+ * zeroed); scratch: aq_tridiag_eigvecs_scratch_doubles() doubles.
+ * aq_hetrd_vblocks / aq_hetrd_tfactors: the vectors v_i in blocks of nb <=
+ * 64 (V [batch, nblk, nb, n], nblk = ceil((n-1)/nb), zero rows past i = n-2)
+ * and each block's forward compact-WY factor T' (nb x nb, from the block's
+ * Gram Gv = conj(V) V^T [batch, nblk, nb, nb]) with
+ * H_{i0}^H ... H_{i0+nb-1}^H = I - V^T T' conj(V): the back-transformation
+ * Et = Zt Q^T is, block by block in increasing order,
+ * Et <- Et - (Et conj(V)^T) (conj(T') V).
+ * aq_orth_band_reduce: err[b] = max |Q - I| over the diagonal Gram tiles Q
+ * [batch, ntile, 64, 128] (Q[t] = rows 64t.. of Et times the conjugate
+ * transpose of rows 64t .. 64t+127), restricted to indices < kvec[b]. */
+int aq_hetrd_batched(int64_t n, void* A, int64_t strideA, double* D, double* E, void* tau,
+                     int64_t batch, hipStream_t stream);
+int aq_tridiag_eigvals(int64_t n, const double* D, int64_t sD, const double* E, int64_t sE,
+                       double* W, int64_t sW, int64_t nval, int64_t batch, hipStream_t stream);
+int64_t aq_tridiag_eigvecs_scratch_doubles(int64_t n, int64_t kmax, int64_t batch);
+int aq_tridiag_eigvecs(int64_t n, const double* D, int64_t sD, const double* E, int64_t sE,
+                       const double* W, int64_t sW, const int32_t* kvec, int64_t kmax, void* Zt,
+                       int64_t sZ, double* scratch, int64_t scratch_doubles, int64_t batch,
+                       hipStream_t stream);
+int aq_hetrd_vblocks(int64_t n, int64_t nb, const void* A, int64_t strideA, void* V,
+                     int64_t batch, hipStream_t stream);
+int aq_hetrd_tfactors(int64_t n, int64_t nb, const void* tau, int64_t stau, const void* Gv,
+                      void* Tf, int64_t batch, hipStream_t stream);
+int aq_orth_band_reduce(const void* Q, int64_t ntile, const int32_t* kvec, int64_t kmax,
+                        double* err, int64_t batch, hipStream_t stream);
 
 /* Rank-predictor MLP 32 -> 128 -> 128 -> 1 with GELU and BF16 weights (no
  * reference counterpart for pre-SVD prediction; the reference's predictors,

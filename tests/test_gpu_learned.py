@@ -57,5 +57,7 @@ def test_trained_network_gain_over_method_without_it():
     for e, l_ in zip(exact, learned):
         _ranks_agree(l_.statistics.logs["k_star"], e.statistics.logs["k_star"])
         assert abs(1 - _slice_fidelity(e, l_)) < 1e-6
+    # deterministic gain: full factorizations avoided (the timings are printed
+    # only: two short wall-clock totals can swap order under queue noise)
     assert cnt(learned, "rsvd") > cnt(nomodel, "rsvd")
-    assert t_learned < t_nomodel
+    assert cnt(learned, "gram") < cnt(nomodel, "gram")

@@ -2,10 +2,22 @@
 sampler against the reference's golden vectors, the CPU oracle and exact
 state vectors.
 
-Tolerances: amplitudes within 1e-5 of the largest amplitude for complex64,
-1e-10 for complex128 (LAPACK vs rocSOLVER factors); sampled bitstrings
-bit-exact for every shot whose draws all clear the oracle's p0 by more than
-1e-5 (complex64) / 1e-10 (complex128).
+Amplitudes: per amplitude, |psi - ref| <= tol * max(|ref|, floor) with tol
+1e-5 for complex64 and 1e-12 for complex128, floor = the root-mean-square
+amplitude of the compared set (amplitudes below it are held to the absolute
+bound at the floor).  The measured maximum is printed.
+
+Kept ranks: equal to the reference's at every update, except where the
+difference is explained by the spectrum itself, detected explicitly on the
+device spectrum of that update and printed: the weight between the two cuts
+lies below the storage precision's resolution of the total (a saturated
+threshold decided by the rounding of tiny singular values), or the values at
+the cut are degenerate (sigma_k / sigma_{k+1} - 1 < 1e-6).  After such an
+update the truncated state is implementation-defined at that weight, and the
+amplitude bound widens by the excluded weight.
+
+Sampled bitstrings: bit-exact for every shot whose draws all clear the
+oracle's p0 by more than 1e-5 (complex64) / 1e-10 (complex128).
 """
 import numpy as np
 import pytest
@@ -20,9 +32,26 @@ from atlas_q import kernels as K
 
 pytestmark = pytest.mark.gpu
 DEV = "cuda"
-AMP_TOL = {torch.complex64: 1e-5, torch.complex128: 1e-10}
+AMP_TOL = {torch.complex64: 1e-5, torch.complex128: 1e-12}
 MARGIN = {torch.complex64: 1e-5, torch.complex128: 1e-10}
 DTYPES = {"torch.complex64": torch.complex64, "torch.complex128": torch.complex128}
+RES = {torch.complex64: 2.0 ** -24, torch.complex128: 2.0 ** -53}
+DEGENERATE = 1e-6
+
+
+class _SpectrumLog:
+    """Records the spectrum each rank-rule call sees (plain method)."""
+
+    def __init__(self, monkeypatch):
+        import atlas_q.adaptive_mps as AM
+        self.spectra = []
+        inner = AM.choose_rank_from_sigma
+
+        def rec(S, eps, cap, budget_ok):
+            out = inner(S, eps, cap, budget_ok)
+            self.spectra.append(S.detach().double().cpu().numpy())
+            return out
+        monkeypatch.setattr(AM, "choose_rank_from_sigma", rec)
 
 
 def _run_brickwork(g, dtype, budget=None, promote=None, **kw):
@@ -36,17 +65,50 @@ def _run_brickwork(g, dtype, budget=None, promote=None, **kw):
     return m
 
 
-def _check_ranks(m, g, dtype):
-    """k sequence equal to the reference's; a difference is allowed only where
-    both discarded tails are below the working precision (a float-saturated
-    threshold resolves on rounding of tiny singular values)."""
+def _explained_rank_difference(S, k_got, k_ref, dtype):
+    """Why two kept ranks on the same update may differ, or None."""
+    w = np.asarray(S, np.float64) ** 2
+    total = w.sum()
+    lo, hi = min(k_got, k_ref), max(k_got, k_ref)
+    between = w[lo:hi].sum()
+    if between <= 16 * RES[dtype] * total:
+        return ("saturated", between / total)
+    s = np.sqrt(w)
+    if hi <= len(s) and s[lo - 1] > 0 and s[lo - 1] / max(s[hi - 1], 1e-300) - 1 < DEGENERATE:
+        return ("degenerate", between / total)
+    return None
+
+
+def _check_ranks(m, g, dtype, spectra):
+    """k sequence equal to the reference's up to explained differences;
+    returns the excluded updates' relative weights between the cuts."""
     ks = m.statistics.logs["k_star"]
-    eps = m.statistics.logs["eps_local"]
-    tail = 1e-3 if dtype == torch.complex64 else 1e-7
-    assert len(ks) == len(g["k"])
+    assert len(ks) == len(g["k"]) == len(spectra)
+    excluded = []
     for j, (k, kg) in enumerate(zip(ks, g["k"].tolist())):
-        if k != kg:
-            assert eps[j] < tail and g["eps_local"][j] < tail, (j, k, kg, eps[j])
+        if k == kg:
+            continue
+        why = _explained_rank_difference(spectra[j], k, kg, dtype)
+        assert why is not None, (j, k, kg, m.statistics.logs["eps_local"][j])
+        print(f"  update {j}: k {k} vs reference {kg} excluded ({why[0]}, "
+              f"weight between cuts {why[1]:.1e})", flush=True)
+        excluded.append(why[1])
+    return excluded
+
+
+def _check_amplitudes(psi, ref, dtype, label, excluded=()):
+    """Per-amplitude relative bound with an RMS magnitude floor; widened by
+    the square root of the excluded weight when updates were excluded."""
+    psi = np.asarray(psi, np.complex128)
+    ref = np.asarray(ref, np.complex128)
+    floor = float(np.sqrt(np.mean(np.abs(ref) ** 2)))
+    scale = np.maximum(np.abs(ref), floor)
+    rel = np.abs(psi - ref) / scale
+    tol = AMP_TOL[dtype] + 4.0 * float(np.sqrt(sum(excluded))) * np.abs(ref).max() / floor
+    print(f"  {label}: max per-amplitude relative error {rel.max():.2e} "
+          f"(tol {tol:.1e}, floor {floor:.2e}, {len(excluded)} excluded updates)", flush=True)
+    assert rel.max() <= tol, (label, rel.max(), tol)
+    return rel.max()
 
 
 def test_budget_run_matches_reference_ranks():
@@ -67,26 +129,27 @@ def test_budget_run_matches_reference_ranks():
                                          ("brick_n10_c64_eps", None),
                                          ("brick_n10_c128_eps", None),
                                          ("config1_n16_c128", None)])
-def test_adaptive_mps_matches_reference_golden(name, budget):
+def test_adaptive_mps_matches_reference_golden(name, budget, monkeypatch):
     print(f"golden run {name}", flush=True)
     g = load_golden(name)
     dtype = DTYPES[str(g["dtype"])]
+    log = _SpectrumLog(monkeypatch)
     m = _run_brickwork(g, dtype, budget)
-    _check_ranks(m, g, dtype)
+    excluded = _check_ranks(m, g, dtype, log.spectra)
     psi = m.to_statevector().cpu().numpy().astype(np.complex128)
     if "psi" in g:
         ref = g["psi"]
     else:
         psi, ref = psi[g["amp_index"]], g["amp"]
-    err = np.abs(psi - ref).max() / np.abs(ref).max()
-    assert err < AMP_TOL[dtype] * 10, err
+    _check_amplitudes(psi, ref, dtype, name, excluded)
     # non-saturated thresholds and cap-bound runs: identical rank sequence
     if name in ("brick_n10_c64_eps", "brick_n10_c128_eps"):
         assert m.statistics.logs["k_star"] == g["k"].tolist()
 
 
-def test_hea_golden_and_single_qubit_convention():
+def test_hea_golden_and_single_qubit_convention(monkeypatch):
     g = load_golden("hea_n8_c64")
+    log = _SpectrumLog(monkeypatch)
     m = AdaptiveMPS(8, bond_dim=1, eps_bond=float(g["eps"]), chi_max_per_bond=int(g["chi"]),
                     device=DEV)
     for op in hea(8, int(g["layers"]), int(g["seed"])):
@@ -96,7 +159,8 @@ def test_hea_golden_and_single_qubit_convention():
         else:
             m.apply_two_site_gate(op[1], U)
     psi = m.to_statevector().cpu().numpy()
-    assert np.abs(psi - g["psi"]).max() < 1e-5
+    excluded = _check_ranks(m, g, torch.complex64, log.spectra)
+    _check_amplitudes(psi, g["psi"], torch.complex64, "hea_n8_c64", excluded)
 
 
 @pytest.mark.parametrize("dtype", [torch.complex64, torch.complex128])
@@ -301,5 +365,4 @@ def test_precision_promotion_matches_reference_golden(method):
     assert m.statistics.logs["k_star"] == g["k"].tolist()
     np.testing.assert_allclose(m.statistics.logs["condS"], g["condS"], rtol=1e-3)
     psi = m.to_statevector().cpu().numpy().astype(np.complex128)
-    err = np.abs(psi - g["psi"]).max() / np.abs(g["psi"]).max()
-    assert err < AMP_TOL[torch.complex64] * 10, err
+    _check_amplitudes(psi, g["psi"], torch.complex64, f"brick_n8_promote ({method})")
