@@ -106,6 +106,7 @@ class AdaptiveMPS(MatrixProductStatePyTorch):
         self.harvest = None          # list -> (features, kept rank) pairs are appended
         self._last_entropy = {}
         self._last_eps = {}
+        self._at_cap = {}      # bond -> the last update there kept the per-bond cap
         self._initialize_zero_state()
 
     def _initialize_zero_state(self):
@@ -288,6 +289,7 @@ class AdaptiveMPS(MatrixProductStatePyTorch):
         cl, cr = A_new.shape[0], B_new.shape[2]
         self._last_entropy[i] = entropy
         self._last_eps[i] = eps_loc
+        self._at_cap[i] = k >= self.chi_max_per_bond[i]
         if not check_entropy_sanity(entropy, cl, cr):
             print(f"Warning: Entropy {entropy:.3f} exceeds physical bound at bond {i}")
         self.tensors[i] = A_new

@@ -181,11 +181,12 @@ def prepare_update(mps, i: int, A: torch.Tensor, B: torch.Tensor, gate: np.ndarr
         return plain_factor(mps, X, cap, budget_ok, local, cl, cr)
     feats = None
     if (method == "learned" and getattr(mps, "harvest", None) is None and cm >= cap
-            and 2 * cap >= min(X.shape)):
-        # the bond already sits at the cap and a randomized sketch as wide as
-        # the cap would cover half the spectrum or more: the exact Gram method
-        # serves the update whatever the network predicts, so the sketch and
-        # the prediction are not paid for
+            and getattr(mps, "_at_cap", {}).get(i, False) and 2 * cap >= min(X.shape)):
+        # the last update of this bond kept the cap (its rule wanted at least
+        # that many values) and a randomized sketch as wide as the cap would
+        # cover half the spectrum or more: the exact Gram method serves the
+        # update whatever the network predicts, so the sketch and the
+        # prediction are not paid for
         st.reject("at_cap")
         return GramJob(X, cap, budget_ok, local, cl, cr, None)
     pred = getattr(mps, "rank_predictor", None)

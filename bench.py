@@ -62,7 +62,9 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=127,
                     help="timed updates (127 = one odd and one even sub-layer of 128 sites)")
-    ap.add_argument("--warmup", type=int, default=64)
+    ap.add_argument("--warmup", type=int, default=127,
+                    help="untimed updates (127 = both sub-layers once: every bond of the "
+                         "slice has had one update before the timed region)")
     ap.add_argument("--chi", type=int, default=1024)
     ap.add_argument("--sites", type=int, default=128)
     ap.add_argument("--low-sites", type=int, default=10,

@@ -93,8 +93,12 @@ def test_tridiag_degenerate_kronecker_grams():
     pass the orthogonality re-check or are flagged by it (the caller then
     takes the per-matrix path), never silently wrong."""
     rng = np.random.default_rng(7)
-    Gs = [_kron_gram(rng, 128), _kron_gram(rng, 256)]
-    ks = [12, 24]
+    for n, k in ((128, 12), (256, 24)):
+        Gs = [_kron_gram(rng, n), _kron_gram(rng, n)]
+        _kron_case(Gs, [k, k])
+
+
+def _kron_case(Gs, ks):
     W, Et, err = _solve(Gs, ks)
     for b, (G, k) in enumerate(zip(Gs, ks)):
         ref_w = np.linalg.eigvalsh(G)[::-1]
