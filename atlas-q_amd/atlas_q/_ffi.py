@@ -96,7 +96,13 @@ SIGNATURES = {
     "aq_bgemm": (c_int, [c_int, c_i64, c_i64, c_i64, ctypes.c_double, c_vp, c_i64, c_i64, c_i64,
                          c_int, c_vp, c_i64, c_i64, c_i64, c_int, ctypes.c_double, c_vp, c_i64,
                          c_i64, c_i64, c_i64, c_vp]),
+    "aq_bgemm_lower": (c_int, [c_i64, c_i64, ctypes.c_double, c_vp, c_i64, c_i64, c_i64, c_int,
+                               c_vp, c_i64, c_i64, c_i64, c_int, ctypes.c_double, c_vp, c_i64,
+                               c_i64, c_i64, c_i64, c_vp]),
     "aq_hetrd_batched": (c_int, [c_i64, c_vp, c_i64, c_vp, c_vp, c_vp, c_i64, c_vp]),
+    "aq_hetrd_workspace_elems": (c_i64, [c_i64, c_i64, c_i64]),
+    "aq_hetrd_own": (c_int, [c_i64, c_i64, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_i64, c_i64,
+                             c_vp]),
     "aq_tridiag_eigvals": (c_int, [c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_i64, c_i64,
                                    c_vp]),
     "aq_tridiag_eigvecs_scratch_doubles": (c_i64, [c_i64, c_i64, c_i64]),

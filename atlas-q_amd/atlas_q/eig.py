@@ -106,6 +106,8 @@ def eigh_desc(G: torch.Tensor):
 
 # ---------------------------------------------------------------- tridiagonal path
 NB_BACK = 64       # Householder vectors per block of the MFMA back-transformation
+HETRD_NB = 32      # panel width of the library's tridiagonal reduction
+REDUCTION = "own"  # "own": the library's kernels; "rocsolver": rocSOLVER's zhetrd
 ORTH_TILE = 64     # rows per diagonal tile of the orthogonality re-check
 ORTH_TOL = 1e-10   # largest |U_k^H U_k - I| entry accepted (else the matrix falls back)
 
@@ -118,7 +120,7 @@ class GramTridiag:
     b by twisted inverse iteration, back-transformed on MFMA, with the FP64
     orthogonality re-check of the result."""
 
-    def __init__(self, Gb: torch.Tensor):
+    def __init__(self, Gb: torch.Tensor, reduction: str = None):
         from . import kernels as K
         from ._ffi import check, lib
         if Gb.dtype != torch.complex128 or not Gb.is_contiguous() or Gb.dim() != 3:
@@ -129,8 +131,20 @@ class GramTridiag:
         self.D = torch.empty((B, n), dtype=torch.float64, device=dev)
         self.E = torch.empty((B, max(n - 1, 1)), dtype=torch.float64, device=dev)
         self.tau = torch.empty((B, max(n - 1, 1)), dtype=torch.complex128, device=dev)
-        check(lib().aq_hetrd_batched(n, K._ptr(Gb), n * n, K._ptr(self.D), K._ptr(self.E),
-                                     K._ptr(self.tau), B, K._stream(Gb)), "aq_hetrd_batched")
+        reduction = reduction or REDUCTION
+        if reduction == "own":
+            L = lib()
+            nws = int(L.aq_hetrd_workspace_elems(n, HETRD_NB, B))
+            ws = torch.empty(nws, dtype=torch.complex128, device=dev)
+            check(L.aq_hetrd_own(n, HETRD_NB, K._ptr(Gb), n * n, K._ptr(self.D), K._ptr(self.E),
+                                 K._ptr(self.tau), K._ptr(ws), nws, B, K._stream(Gb)),
+                  "aq_hetrd_own")
+            self._ws = ws
+        elif reduction == "rocsolver":
+            check(lib().aq_hetrd_batched(n, K._ptr(Gb), n * n, K._ptr(self.D), K._ptr(self.E),
+                                         K._ptr(self.tau), B, K._stream(Gb)), "aq_hetrd_batched")
+        else:
+            raise ValueError(f"unknown tridiagonal reduction {reduction!r}")
         self.W = None
 
     def eigenvalues(self) -> torch.Tensor:

@@ -201,6 +201,14 @@ int aq_bgemm(int dtype, int64_t M, int64_t N, int64_t K, double alpha, const voi
              int64_t b_s1, int64_t b_sb, int conj_b, double beta, void* C, int64_t c_s0,
              int64_t c_s1, int64_t c_sb, int64_t batch, hipStream_t stream);
 
+/* aq_bgemm for complex128 with M == N writing only C's lower triangle (row
+ * >= column; tiles strictly above it are skipped): the rank-2nb updates of
+ * the tridiagonal reduction below, which read only that triangle. */
+int aq_bgemm_lower(int64_t M, int64_t K, double alpha, const void* A, int64_t a_s0,
+                   int64_t a_s1, int64_t a_sb, int conj_a, const void* B, int64_t b_s0,
+                   int64_t b_s1, int64_t b_sb, int conj_b, double beta, void* C, int64_t c_s0,
+                   int64_t c_s1, int64_t c_sb, int64_t batch, hipStream_t stream);
+
 /* Gram eigensolver through the tridiagonal form, in FP64 (no reference
  * counterpart: the reference factors theta with a thin SVD,
  * adaptive_mps.py:249-250 via linalg_robust.py:17-52; the Gram method needs
@@ -231,6 +239,19 @@ int aq_bgemm(int dtype, int64_t M, int64_t N, int64_t K, double alpha, const voi
  * transpose of rows 64t .. 64t+127), restricted to indices < kvec[b]. */
 int aq_hetrd_batched(int64_t n, void* A, int64_t strideA, double* D, double* E, void* tau,
                      int64_t batch, hipStream_t stream);
+/* The same reduction by the library's own kernels (same outputs and storage
+ * as aq_hetrd_batched; LAPACK's blocked zhetrd/zlatrd with uplo "U"): panels
+ * of nb <= 32 columns; per column one batched streaming Hermitian
+ * matrix-vector product over the stored triangle (which also yields the
+ * quadratic form and the panel dot products), one wide kernel forming the W
+ * column and updating the next column, and one per-matrix reflector kernel;
+ * per panel a rank-2nb update of the lower triangle on MFMA
+ * (aq_bgemm_lower).  Batches of 8 or more run as two interleaved groups on
+ * the library's worker streams, joined to `stream` by events.
+ * ws: aq_hetrd_workspace_elems(n, nb, batch) complex128. */
+int64_t aq_hetrd_workspace_elems(int64_t n, int64_t nb, int64_t batch);
+int aq_hetrd_own(int64_t n, int64_t nb, void* A, int64_t strideA, double* D, double* E,
+                 void* tau, void* ws, int64_t ws_elems, int64_t batch, hipStream_t stream);
 int aq_tridiag_eigvals(int64_t n, const double* D, int64_t sD, const double* E, int64_t sE,
                        double* W, int64_t sW, int64_t nval, int64_t batch, hipStream_t stream);
 int64_t aq_tridiag_eigvecs_scratch_doubles(int64_t n, int64_t kmax, int64_t batch);
