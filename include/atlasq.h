@@ -252,6 +252,31 @@ int aq_hetrd_batched(int64_t n, void* A, int64_t strideA, double* D, double* E, 
 int64_t aq_hetrd_workspace_elems(int64_t n, int64_t nb, int64_t batch);
 int aq_hetrd_own(int64_t n, int64_t nb, void* A, int64_t strideA, double* D, double* E,
                  void* tau, void* ws, int64_t ws_elems, int64_t batch, hipStream_t stream);
+/* First stage of the two-stage tridiagonalisation (the library's own; no
+ * reference counterpart, as for the reduction above): Hermitian A [batch]
+ * [n][n] (row-major, lower triangle read; overwritten) = Q1 B Q1^H with B of
+ * bandwidth 32, by panels of 32 Householder reflectors (LAPACK's
+ * zhetrd_he2hb, lower) whose trailing updates run on the batched MFMA GEMM.
+ * On return AB[b][c][d] = B(c + d, c) for d < 64 (zero for d > 32), the
+ * reflectors of panel p (columns 32p .. 32p+31) lie below its R factor in A
+ * (A(32p + 32 + r, 32p + c), r > c, unit diagonal implied) and its T factor
+ * (Q_p = I - V T V^H) at Tall[b][p][32][32].  aq_he2hb_panels(n) panels;
+ * ws: aq_he2hb_workspace_elems(n, batch) complex128; n - 32 <= 4096. */
+int64_t aq_he2hb_panels(int64_t n);
+int64_t aq_he2hb_workspace_elems(int64_t n, int64_t batch);
+int aq_he2hb_own(int64_t n, void* A, int64_t strideA, void* AB, void* Tall, void* ws,
+                 int64_t ws_elems, int64_t batch, hipStream_t stream);
+/* Second stage: the band AB (as aq_he2hb_own leaves it, bandwidth 32) to real
+ * tridiagonal form by bulge chasing, one work-group per matrix (sweeps of
+ * one-column Householder annihilations, 8 sweeps in flight).  D [batch][n],
+ * E [batch][n-1]; every reflector (v of 32 entries, v[0] = 1, and tau) of
+ * sweep i, task s is logged at Vlog[b][i * ceil((n-1)/32) + s][32] and
+ * Tlog[b][i * ceil((n-1)/32) + s], aq_hb2st_log_tasks(n) slots per matrix;
+ * T = Q2^H B Q2 with Q2 the product of the reflectors in sweep order.
+ * 2 <= n <= 4033. */
+int64_t aq_hb2st_log_tasks(int64_t n);
+int aq_hb2st_own(int64_t n, void* AB, void* Vlog, void* Tlog, double* D, double* E,
+                 int64_t batch, hipStream_t stream);
 int aq_tridiag_eigvals(int64_t n, const double* D, int64_t sD, const double* E, int64_t sE,
                        double* W, int64_t sW, int64_t nval, int64_t batch, hipStream_t stream);
 int64_t aq_tridiag_eigvecs_scratch_doubles(int64_t n, int64_t kmax, int64_t batch);
