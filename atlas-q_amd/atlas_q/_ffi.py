@@ -108,6 +108,10 @@ SIGNATURES = {
     "aq_he2hb_own": (c_int, [c_i64, c_vp, c_i64, c_vp, c_vp, c_vp, c_i64, c_i64, c_vp]),
     "aq_hb2st_log_tasks": (c_i64, [c_i64]),
     "aq_hb2st_own": (c_int, [c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_vp]),
+    "aq_geqrf_panels": (c_i64, [c_i64, c_i64]),
+    "aq_geqrf_workspace_elems": (c_i64, [c_i64, c_i64, c_i64]),
+    "aq_geqrf": (c_int, [c_i64, c_i64, c_vp, c_i64, c_vp, c_vp, c_i64, c_i64, c_vp]),
+    "aq_ungqr": (c_int, [c_i64, c_i64, c_vp, c_i64, c_vp, c_vp, c_i64, c_vp, c_i64, c_i64, c_vp]),
     "aq_tridiag_eigvals": (c_int, [c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_i64, c_i64,
                                    c_vp]),
     "aq_tridiag_eigvecs_scratch_doubles": (c_i64, [c_i64, c_i64, c_i64]),

@@ -414,3 +414,38 @@ def grouped_gemm(problems, dtype: torch.dtype, device) -> None:
     grouped_gemm_descs(descs, dtype, device)
 
 
+
+
+QR_MAX_ROWS = 4096   # rows of a factored matrix (the panel kernel keeps one column in LDS)
+
+
+def qr(X: torch.Tensor) -> Tuple[torch.Tensor, torch.Tensor]:
+    """Reduced QR of a complex matrix [m, n] or batch [b, m, n] on the device
+    by the library's blocked Householder factorisation (aq_geqrf + aq_ungqr,
+    LAPACK's zgeqrf / zungqr): Q [.., m, k], R [.., k, n], k = min(m, n), in
+    X's dtype.  The factorisation runs in complex128 (complex64 input is
+    promoted and the factors rounded back); as LAPACK's, R has a real
+    diagonal of either sign."""
+    _dev(X, "X")
+    if X.dtype not in (torch.complex64, torch.complex128) or X.dim() not in (2, 3):
+        raise ValueError(f"qr: need a complex [m, n] or [b, m, n] tensor, got {tuple(X.shape)} "
+                         f"{X.dtype}")
+    batched = X.dim() == 3
+    A = (X if batched else X.unsqueeze(0)).to(torch.complex128).contiguous().clone()
+    b, m, n = A.shape
+    if m > QR_MAX_ROWS:
+        raise ValueError(f"qr: {m} rows exceed the factorisation's limit of {QR_MAX_ROWS}")
+    k = min(m, n)
+    L = lib()
+    npan = int(L.aq_geqrf_panels(m, n))
+    T = torch.empty((b, npan, 32, 32), dtype=torch.complex128, device=A.device)
+    nws = int(L.aq_geqrf_workspace_elems(m, n, b))
+    ws = torch.empty(nws, dtype=torch.complex128, device=A.device)
+    st = _stream(A)
+    check(L.aq_geqrf(m, n, _ptr(A), m * n, _ptr(T), _ptr(ws), nws, b, st), "aq_geqrf")
+    Q = torch.empty((b, m, k), dtype=torch.complex128, device=A.device)
+    check(L.aq_ungqr(m, n, _ptr(A), m * n, _ptr(T), _ptr(Q), m * k, _ptr(ws), nws, b, st),
+          "aq_ungqr")
+    R = torch.triu(A[:, :k, :])
+    Q, R = Q.to(X.dtype), R.to(X.dtype)
+    return (Q, R) if batched else (Q[0], R[0])

@@ -1,8 +1,9 @@
 """SVD / QR with the reference's retry cascade (``src/atlas_q/linalg_robust.py``).
 
-The factorizations run on the HIP device (rocSOLVER through torch).  The
-reference's last resort -- a CPU SVD/QR (linalg_robust.py:49-52, 72-76) -- is
-replaced by a third device attempt with a relative jitter of 1e-9 max|X|
+The SVD runs on the HIP device (rocSOLVER through torch: it is the plain
+method by definition), the QR through the library's Householder kernels.
+The reference's last resort -- a CPU SVD/QR (linalg_robust.py:49-52, 72-76) --
+is replaced by a third device attempt with a relative jitter of 1e-9 max|X|
 (rocSOLVER's SVD can fail to converge on exactly degenerate spectra, e.g.
 product-state blocks of CZ-ladder circuits, where the reference's absolute
 1e-12 jitter is not enough); a failure after it raises.
@@ -44,10 +45,15 @@ def robust_svd(X: torch.Tensor) -> Tuple[torch.Tensor, torch.Tensor, torch.Tenso
 
 
 def robust_qr(X: torch.Tensor) -> Tuple[torch.Tensor, torch.Tensor, str]:
-    """linalg_robust.py:55-76 without the CPU step: (Q, R, driver)."""
+    """linalg_robust.py:55-76: (Q, R, driver), by the library's blocked
+    Householder QR on the device (``kernels.qr``; LAPACK's zgeqrf/zungqr
+    algorithm, so the factors are the ones the reference's torch.linalg.qr
+    returns up to rounding).  Householder QR does not fail on rank-deficient
+    input, so the reference's retry cascade has nothing to retry."""
     _require_device(X)
-    Q, R = torch.linalg.qr(X)
-    return Q, R, "torch_cuda"
+    from . import kernels as K
+    Q, R = K.qr(X)
+    return Q, R, "atlas_householder"
 
 
 def condition_number(S: torch.Tensor) -> float:

@@ -95,7 +95,7 @@ class MatrixProductStatePyTorch(CompressedQuantumStatePyTorch):
         for i in range(self.num_qubits - 1):
             T = self.tensors[i]
             l, p, r = T.shape
-            Q, R = torch.linalg.qr(T.reshape(l * p, r))
+            Q, R = K.qr(T.reshape(l * p, r))
             self.tensors[i] = Q.reshape(l, p, Q.shape[1])
             self._absorb_right(i + 1, R)
         self.is_canonical = True
@@ -105,7 +105,7 @@ class MatrixProductStatePyTorch(CompressedQuantumStatePyTorch):
         for i in range(self.num_qubits - 1, 0, -1):
             T = self.tensors[i]
             l, p, r = T.shape
-            Q, R = torch.linalg.qr(T.reshape(l, p * r).T)
+            Q, R = K.qr(T.reshape(l, p * r).T)
             Q, R = Q.T, R.T
             self.tensors[i] = Q.reshape(Q.shape[0], p, r)
             self._absorb_left(i - 1, R)

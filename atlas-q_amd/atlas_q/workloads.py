@@ -35,7 +35,11 @@ def random_right_isometry(chi_l: int, chi_r: int, gen: torch.Generator) -> torch
     dev = gen.device
     Z = torch.complex(torch.randn(2 * chi_r, chi_l, generator=gen, dtype=torch.float64, device=dev),
                       torch.randn(2 * chi_r, chi_l, generator=gen, dtype=torch.float64, device=dev))
-    Q, _ = torch.linalg.qr(Z)
+    if dev.type == "cuda":
+        from . import kernels as K
+        Q, _ = K.qr(Z)                 # the library's Householder QR
+    else:
+        Q, _ = torch.linalg.qr(Z)      # host setup draws (LAPACK on the host)
     return Q.T.conj().reshape(chi_l, 2, chi_r)
 
 

@@ -1,5 +1,14 @@
-// Batched reduction of Hermitian matrices to band form (FP64), the first
-// stage of the two-stage tridiagonalisation: A = Q1 B Q1^H with B Hermitian
+// Householder factorisations in FP64 on the batched MFMA GEMM, built on one
+// panel kernel (panel_qr_kernel: zgeqr2 + zlarft on up to 32 columns, one
+// work-group per matrix):
+//   * the band reduction, first stage of the two-stage tridiagonalisation
+//     (aq_he2hb_own, below);
+//   * the blocked QR factorisation and the thin Q (aq_geqrf / aq_ungqr, at the
+//     end of the file; LAPACK's zgeqrf / zungqr, so R and Q match LAPACK's
+//     up to rounding: R's diagonal is real, tau from zlarfg).
+//
+// Band reduction: batched reduction of Hermitian matrices to band form, the
+// first stage of the two-stage tridiagonalisation: A = Q1 B Q1^H with B Hermitian
 // of bandwidth 32 (B(r, c) = 0 for |r - c| > 32).
 //
 // Panels of 32 columns, left to right (LAPACK's zhetrd_he2hb, uplo = lower):
@@ -65,11 +74,12 @@ __global__ void sym_fill_kernel(double2* __restrict__ A, int64_t sA, int n) {
   }
 }
 
-// Householder QR of one 32-column panel per matrix (zgeqr2 + zlarft).
-// P(r, c) = P[r ld + c], r < m.  On return: R in P's upper triangle, v_c[r]
-// (r > c) below it; V (unit diagonal, zeros above) at V[r ldv + c]; its
-// conjugate transpose at Vh[c ldh + r]; T (32 x 32 upper) at Tf[c * 32 + c'].
-// Columns c >= m (a short last panel) get tau = 0 and v = 0.
+// Householder QR of one panel of ncol <= 32 columns per matrix (zgeqr2 +
+// zlarft).  P(r, c) = P[r ld + c], r < m, c < ncol.  On return: R in P's upper
+// triangle, v_c[r] (r > c) below it; V (unit diagonal, zeros above) at
+// V[r ldv + c]; its conjugate transpose at Vh[c ldh + r]; T (32 x 32 upper) at
+// Tf[c * 32 + c'].  Columns c >= min(m, ncol) get tau = 0 and v = 0 (V, Vh and
+// T always have 32 columns).
 //
 // Thread (g, col) = (tid / 32, tid % 32) owns the rows r = g mod 32 of column
 // col.  Per reflector c three passes: dots (v_c^H P(:, c') for c' > c, and
@@ -92,7 +102,7 @@ __device__ __forceinline__ void qr_reflector(double2 alpha, double xn2, double2&
 }
 
 __global__ __launch_bounds__(kQRT) void panel_qr_kernel(double2* __restrict__ P, int64_t sP,
-                                                        int64_t ld, int m,
+                                                        int64_t ld, int m, int ncol,
                                                         double2* __restrict__ V, int64_t sV,
                                                         int64_t ldv, double2* __restrict__ Vh,
                                                         int64_t sVh, int64_t ldh,
@@ -126,14 +136,14 @@ __global__ __launch_bounds__(kQRT) void panel_qr_kernel(double2* __restrict__ P,
       for (int q = 0; q < 32; ++q) xn2 += red[q][0].x;
       double2 tau = zero, scale = make_double2(1.0, 0.0);
       double beta = 0.0;
-      if (c < m) qr_reflector(Pb[(int64_t)c * ld + c], xn2, tau, scale, beta);
+      if (c < m && c < ncol) qr_reflector(Pb[(int64_t)c * ld + c], xn2, tau, scale, beta);
       s_tau = tau;
       s_scale = scale;
       s_beta = beta;
       Ts[c][c] = tau;
     }
     __syncthreads();
-    if (c >= m) continue;  // uniform: nothing left to annihilate
+    if (c >= m || c >= ncol) continue;  // uniform: nothing left to annihilate
     const double2 tau = s_tau;
     {
       const double2 scale = s_scale;
@@ -153,7 +163,7 @@ __global__ __launch_bounds__(kQRT) void panel_qr_kernel(double2* __restrict__ P,
     // (2) dots over rows r >= c
     {
       double2 s = zero;
-      if (col != c) {
+      if (col != c && col < ncol) {
         const double sg = col > c ? -1.0 : 1.0;  // conj(v) x  or  conj(x) v
         int r = first_row(c, g);
         for (; r + 7 * 32 < m; r += 8 * 32) {
@@ -192,7 +202,7 @@ __global__ __launch_bounds__(kQRT) void panel_qr_kernel(double2* __restrict__ P,
     // (3) P(:, c') -= conj(tau) v (v^H P(:, c')), c' > c; column c+1 also
     // sums the norm of its rows > c+1
     double part = 0.0;
-    if (col > c) {
+    if (col > c && col < ncol) {
       const double2 coef = cmul(dconj(tau), wv[col]);
       const bool nxt = col == c + 1;
       int r = first_row(c, g);
@@ -227,8 +237,10 @@ __global__ __launch_bounds__(kQRT) void panel_qr_kernel(double2* __restrict__ P,
   double2* Vhb = Vh + b * sVh;
   for (int r = g; r < m; r += 32) {
     double2 x = zero;
-    if (r == col) x = make_double2(1.0, 0.0);
-    else if (r > col) x = Pb[(int64_t)r * ld + col];
+    if (col < ncol) {
+      if (r == col) x = make_double2(1.0, 0.0);
+      else if (r > col) x = Pb[(int64_t)r * ld + col];
+    }
     Vb[(int64_t)r * ldv + col] = x;
     Vhb[(int64_t)col * ldh + r] = dconj(x);
   }
@@ -500,8 +512,8 @@ extern "C" int aq_he2hb_own(int64_t n, void* Av, int64_t strideA, void* ABv, voi
       double2* Pp = g[q].A + (j + kPB) * n + j;
       double2* A22 = g[q].A + (j + kPB) * n + (j + kPB);
       double2* Tp = g[q].Tall + p * kPB * kPB;
-      panel_qr_kernel<<<(unsigned)bq, kQRT, 0, s>>>(Pp, strideA, n, (int)m, g[q].Cat + kPB, sCat,
-                                                    ldc, g[q].Vh, sVh, n, Tp, sT);
+      panel_qr_kernel<<<(unsigned)bq, kQRT, 0, s>>>(Pp, strideA, n, (int)m, kPB, g[q].Cat + kPB,
+                                                    sCat, ldc, g[q].Vh, sVh, n, Tp, sT);
       AQ_LAUNCH_CHECK();
       // Yt = V^H A22  [32 x m]
       int st = aq_bgemm(AQ_C128, kPB, m, m, 1.0, g[q].Vh, n, 1, sVh, 0, A22, n, 1, strideA, 0, 0.0,
@@ -532,6 +544,133 @@ extern "C" int aq_he2hb_own(int64_t n, void* Av, int64_t strideA, void* ABv, voi
       AQ_HIP(hipStreamWaitEvent(stream, ev, 0));
     }
     AQ_HIP(hipEventDestroy(ev));
+  }
+  return AQ_OK;
+}
+
+// ------------------------------------------------------------ QR (zgeqrf, zungqr)
+namespace aq {
+// V (unit diagonal, zeros above) and V^H of panel columns j .. j+31 from the
+// factored A (rows j .. m-1, row stride lda): the form the panel kernel
+// leaves, rebuilt for the Q formation.
+__global__ void extract_v_kernel(const double2* __restrict__ A, int64_t sA, int64_t lda,
+                                 int mp, int ncol, double2* __restrict__ V, int64_t sV,
+                                 double2* __restrict__ Vh, int64_t sVh, int64_t ldh) {
+  const int64_t b = blockIdx.y;
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)mp * kPB) return;
+  const int r = (int)(idx / kPB), c = (int)(idx % kPB);
+  double2 x = make_double2(0.0, 0.0);
+  if (c < ncol && r >= c) x = r == c ? make_double2(1.0, 0.0) : A[b * sA + (int64_t)r * lda + c];
+  V[b * sV + idx] = x;
+  Vh[b * sVh + (int64_t)c * ldh + r] = dconj(x);
+}
+
+// Q = [I_k; 0] (m x k, row stride k)
+__global__ void eye_kernel(double2* __restrict__ Q, int64_t sQ, int m, int k) {
+  const int64_t b = blockIdx.y;
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)m * k) return;
+  const int r = (int)(idx / k), c = (int)(idx % k);
+  Q[b * sQ + idx] = make_double2(r == c ? 1.0 : 0.0, 0.0);
+}
+}  // namespace aq
+
+extern "C" int64_t aq_geqrf_panels(int64_t m, int64_t n) { return cdiv(std::min(m, n), kPB); }
+
+extern "C" int64_t aq_geqrf_workspace_elems(int64_t m, int64_t n, int64_t batch) {
+  // per matrix: V [m][32] | V^H [32][m] | W, W2 [32][max(m, n)]
+  return batch * (2 * kPB * m + 2 * kPB * std::max(m, n));
+}
+
+extern "C" int aq_geqrf(int64_t m, int64_t n, void* Av, int64_t strideA, void* Tallv, void* ws,
+                        int64_t ws_elems, int64_t batch, hipStream_t stream) {
+  AQ_REQUIRE(Av && Tallv && ws, AQ_ERR_ARG, "aq_geqrf: null pointer");
+  AQ_REQUIRE(m >= 1 && n >= 1 && m <= kQRMaxRows, AQ_ERR_ARG,
+             "aq_geqrf: shape %lld x %lld out of range (rows <= %d)", (long long)m, (long long)n,
+             kQRMaxRows);
+  AQ_REQUIRE(batch >= 1 && batch <= 65535 && strideA >= m * n, AQ_ERR_ARG,
+             "aq_geqrf: bad batch or stride");
+  AQ_REQUIRE(ws_elems >= aq_geqrf_workspace_elems(m, n, batch), AQ_ERR_ARG,
+             "aq_geqrf: workspace too small");
+  double2* A = (double2*)Av;
+  double2* Tall = (double2*)Tallv;
+  const int64_t npan = aq_geqrf_panels(m, n), sT = npan * kPB * kPB;
+  const int64_t wmax = std::max(m, n);
+  const int64_t sV = kPB * m, sW = kPB * wmax;
+  double2* V = (double2*)ws;
+  double2* Vh = V + batch * sV;
+  double2* W = Vh + batch * sV;
+  double2* W2 = W + batch * sW;
+  for (int64_t p = 0; p < npan; ++p) {
+    const int64_t j = p * kPB, mp = m - j, nc = std::min<int64_t>(kPB, n - j);
+    const int64_t ntr = n - j - nc;
+    double2* Tp = Tall + p * kPB * kPB;
+    panel_qr_kernel<<<(unsigned)batch, kQRT, 0, stream>>>(A + j * n + j, strideA, n, (int)mp,
+                                                          (int)nc, V, sV, kPB, Vh, sV, m, Tp, sT);
+    AQ_LAUNCH_CHECK();
+    if (ntr <= 0) continue;
+    double2* Atr = A + j * n + j + nc;
+    // W = V^H A_tr ; W2 = T^H W ; A_tr -= V W2
+    int st = aq_bgemm(AQ_C128, kPB, ntr, mp, 1.0, Vh, m, 1, sV, 0, Atr, n, 1, strideA, 0, 0.0, W,
+                      ntr, 1, sW, batch, stream);
+    if (st) return st;
+    st = aq_bgemm(AQ_C128, kPB, ntr, kPB, 1.0, Tp, 1, kPB, sT, 1, W, ntr, 1, sW, 0, 0.0, W2, ntr,
+                  1, sW, batch, stream);
+    if (st) return st;
+    st = aq_bgemm(AQ_C128, mp, ntr, kPB, -1.0, V, kPB, 1, sV, 0, W2, ntr, 1, sW, 0, 1.0, Atr, n, 1,
+                  strideA, batch, stream);
+    if (st) return st;
+  }
+  return AQ_OK;
+}
+
+extern "C" int aq_ungqr(int64_t m, int64_t n, const void* Av, int64_t strideA, const void* Tallv,
+                        void* Qv, int64_t strideQ, void* ws, int64_t ws_elems, int64_t batch,
+                        hipStream_t stream) {
+  AQ_REQUIRE(Av && Tallv && Qv && ws, AQ_ERR_ARG, "aq_ungqr: null pointer");
+  AQ_REQUIRE(m >= 1 && n >= 1 && m <= kQRMaxRows, AQ_ERR_ARG, "aq_ungqr: shape out of range");
+  const int64_t k = std::min(m, n);
+  AQ_REQUIRE(batch >= 1 && batch <= 65535 && strideA >= m * n && strideQ >= m * k, AQ_ERR_ARG,
+             "aq_ungqr: bad batch or stride");
+  AQ_REQUIRE(ws_elems >= aq_geqrf_workspace_elems(m, n, batch), AQ_ERR_ARG,
+             "aq_ungqr: workspace too small");
+  const double2* A = (const double2*)Av;
+  const double2* Tall = (const double2*)Tallv;
+  double2* Q = (double2*)Qv;
+  const int64_t npan = aq_geqrf_panels(m, n), sT = npan * kPB * kPB;
+  const int64_t wmax = std::max(m, n);
+  const int64_t sV = kPB * m, sW = kPB * wmax;
+  double2* V = (double2*)ws;
+  double2* Vh = V + batch * sV;
+  double2* W = Vh + batch * sV;
+  double2* W2 = W + batch * sW;
+  {
+    dim3 grid((unsigned)cdiv(m * k, 256), (unsigned)batch);
+    eye_kernel<<<grid, 256, 0, stream>>>(Q, strideQ, (int)m, (int)k);
+    AQ_LAUNCH_CHECK();
+  }
+  // Q = H_0 H_1 ... [I; 0]: panels applied last to first, each to columns j .. k-1
+  for (int64_t p = npan - 1; p >= 0; --p) {
+    const int64_t j = p * kPB, mp = m - j, nc = std::min<int64_t>(kPB, n - j), kc = k - j;
+    const double2* Tp = Tall + p * kPB * kPB;
+    {
+      dim3 grid((unsigned)cdiv(mp * kPB, 256), (unsigned)batch);
+      extract_v_kernel<<<grid, 256, 0, stream>>>(A + j * n + j, strideA, n, (int)mp, (int)nc, V,
+                                                 sV, Vh, sV, m);
+      AQ_LAUNCH_CHECK();
+    }
+    double2* Qs = Q + j * k + j;
+    // W = V^H Q_s ; W2 = T W ; Q_s -= V W2
+    int st = aq_bgemm(AQ_C128, kPB, kc, mp, 1.0, Vh, m, 1, sV, 0, Qs, k, 1, strideQ, 0, 0.0, W, kc,
+                      1, sW, batch, stream);
+    if (st) return st;
+    st = aq_bgemm(AQ_C128, kPB, kc, kPB, 1.0, Tp, kPB, 1, sT, 0, W, kc, 1, sW, 0, 0.0, W2, kc, 1,
+                  sW, batch, stream);
+    if (st) return st;
+    st = aq_bgemm(AQ_C128, mp, kc, kPB, -1.0, V, kPB, 1, sV, 0, W2, kc, 1, sW, 0, 1.0, Qs, k, 1,
+                  strideQ, batch, stream);
+    if (st) return st;
   }
   return AQ_OK;
 }

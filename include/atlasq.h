@@ -277,6 +277,22 @@ int aq_he2hb_own(int64_t n, void* A, int64_t strideA, void* AB, void* Tall, void
 int64_t aq_hb2st_log_tasks(int64_t n);
 int aq_hb2st_own(int64_t n, void* AB, void* Vlog, void* Tlog, double* D, double* E,
                  int64_t batch, hipStream_t stream);
+/* Blocked Householder QR of complex128 matrices A [batch][m][n] (row-major,
+ * m <= 4096), LAPACK's zgeqrf: panels of 32 columns factored by one
+ * work-group each (zgeqr2 + zlarft), trailing updates on the batched GEMM.
+ * Replaces the device QR behind the reference's robust_qr
+ * (linalg_robust.py:55-76; torch.linalg.qr there) and the canonical forms'
+ * QRs (adaptive_mps.py:310-385, mps_pytorch.py:91-152).  On return A holds R
+ * in its upper triangle and the reflectors below it; Tall[b][p][32][32] the
+ * T factor of panel p (aq_geqrf_panels(m, n) panels).  aq_ungqr forms the
+ * thin Q [batch][m][min(m, n)] (row stride min(m, n)) from them (zungqr).
+ * ws: aq_geqrf_workspace_elems(m, n, batch) complex128, for either call. */
+int64_t aq_geqrf_panels(int64_t m, int64_t n);
+int64_t aq_geqrf_workspace_elems(int64_t m, int64_t n, int64_t batch);
+int aq_geqrf(int64_t m, int64_t n, void* A, int64_t strideA, void* Tall, void* ws,
+             int64_t ws_elems, int64_t batch, hipStream_t stream);
+int aq_ungqr(int64_t m, int64_t n, const void* A, int64_t strideA, const void* Tall, void* Q,
+             int64_t strideQ, void* ws, int64_t ws_elems, int64_t batch, hipStream_t stream);
 int aq_tridiag_eigvals(int64_t n, const double* D, int64_t sD, const double* E, int64_t sE,
                        double* W, int64_t sW, int64_t nval, int64_t batch, hipStream_t stream);
 int64_t aq_tridiag_eigvecs_scratch_doubles(int64_t n, int64_t kmax, int64_t batch);
