@@ -112,6 +112,17 @@ ORTH_TILE = 64     # rows per diagonal tile of the orthogonality re-check
 ORTH_TOL = 1e-10   # largest |U_k^H U_k - I| entry accepted (else the matrix falls back)
 
 
+def reduction_bytes(n: int, nb: int = HETRD_NB) -> float:
+    """Algorithmic HBM bytes of the one-stage blocked reduction of one order-n
+    matrix: every column's Hermitian matrix-vector product reads the stored
+    triangle of the leading ii x ii block once (ii(ii+1)/2 complex128), and
+    every panel's rank-2nb update reads and writes the lower triangle of the
+    leading p0 x p0 block."""
+    hemv = sum(ii * (ii + 1) // 2 for ii in range(1, n)) * 16.0
+    upd = sum(p0 * (p0 + 1) // 2 for p0 in range(n - nb, 0, -nb)) * 32.0
+    return hemv + upd
+
+
 class GramTridiag:
     """Leading eigenpairs of a batch of Hermitian complex128 Gram cores
     [B, n, n] (row-major, lower triangle read, overwritten) through the

@@ -350,7 +350,7 @@ def _gram_group_tridiag(mps, jobs, st):
     path.  The projection reads the back-transformed rows, which are the rows
     of U_k^H."""
     from ._ffi import AQ_GROUPED_HERK, COPY_DESC, check, lib
-    from .eig import ORTH_TOL, GramTridiag
+    from .eig import ORTH_TOL, GramTridiag, reduction_bytes
     c128 = torch.complex128
     B = len(jobs)
     m = jobs[0].X.shape[0]
@@ -363,9 +363,11 @@ def _gram_group_tridiag(mps, jobs, st):
     work = float(sum(4.0 * m * m * x.shape[1] for x in X64))
     with RL.timed("gram_fp64_herk_grouped", work, "fp64_mfma"):
         K.grouped_gemm_descs(gd, c128, dev, keep)
-    # algorithmic flops: complex Hermitian tridiagonalisation 16/3 n^3; the
-    # Sturm counts of bisection ~53 steps of 4 flops per entry and eigenvalue
-    with RL.timed("eig_zhetrd_batched", B * 16.0 / 3.0 * m ** 3, "fp64_mfma"):
+    # algorithmic flops: complex Hermitian tridiagonalisation 16/3 n^3 (and the
+    # one-stage method's bytes, which bound it); the Sturm counts of
+    # bisection ~53 steps of 4 flops per entry and eigenvalue
+    with RL.timed("eig_zhetrd_batched", B * 16.0 / 3.0 * m ** 3, "fp64_mfma",
+                  nbytes=B * reduction_bytes(m)):
         tri = GramTridiag(Gb)
     with RL.timed("eig_bisection", B * 53.0 * 4.0 * m * m, "fp64_valu"):
         lam = tri.eigenvalues()

@@ -33,7 +33,10 @@ def reset() -> None:
 
 @contextlib.contextmanager
 def timed(name: str, work: float, bound: str, nbytes: float = 0.0):
-    """Time one launch.  work: algorithmic flops (MFMA bounds) or bytes (hbm)."""
+    """Time one launch.  work: algorithmic flops (MFMA / VALU bounds) or bytes
+    (hbm); nbytes: the algorithmic bytes of a flop-counted launch, which make
+    it HBM-bound when moving them at 8 TB/s takes longer than the flops at
+    their peak."""
     if not _enabled:
         yield
         return
@@ -72,20 +75,34 @@ def summary(window_s: float = None, units: int = None):
         if bound == "hbm":
             achieved = per / (mean_ms * 1e-3) / 1e9
             ideal = work / (PEAK["hbm"] * 1e9)
+            ent = {"bound": bound, "achieved": achieved, "peak": PEAK[bound], "unit": UNIT[bound],
+                   "frac": achieved / PEAK[bound]}
         else:
-            achieved = per / (mean_ms * 1e-3) / 1e12
-            ideal = max(work / (PEAK[bound] * 1e12), nb / (PEAK["hbm"] * 1e9))
+            t_flops = work / (PEAK[bound] * 1e12)
+            t_bytes = nb / (PEAK["hbm"] * 1e9)
+            ideal = max(t_flops, t_bytes)
+            flops_rate = per / (mean_ms * 1e-3) / 1e12
+            ent = {"bound": bound, "achieved": flops_rate, "peak": PEAK[bound],
+                   "unit": UNIT[bound], "frac": flops_rate / PEAK[bound]}
+            if t_bytes > t_flops:
+                # the algorithm's bytes bound it: report against HBM
+                bw = nb / cnt / (mean_ms * 1e-3) / 1e9
+                ent = {"bound": "hbm", "achieved": bw, "peak": PEAK["hbm"], "unit": "GB/s",
+                       "frac": bw / PEAK["hbm"], "bytes_per_launch": nb / cnt,
+                       "flops_achieved": flops_rate, "flops_unit": UNIT[bound],
+                       "flops_frac": flops_rate / PEAK[bound]}
         ideal_s += ideal
-        kernels[name] = {"launches": cnt, "mean_ms": mean_ms, "work_per_launch": per,
-                         "bound": bound, "achieved": achieved, "peak": PEAK[bound],
-                         "unit": UNIT[bound], "frac": achieved / PEAK[bound],
-                         "total_ms": ms, "ideal_ms": ideal * 1e3}
+        kernels[name] = dict(launches=cnt, mean_ms=mean_ms, work_per_launch=per, total_ms=ms,
+                             ideal_ms=ideal * 1e3, **ent)
     dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
     d = kernels[dom]
     out = {"kernel": dom, "bound": "hbm" if d["bound"] == "hbm" else "mfma",
            "achieved": d["achieved"], "peak": d["peak"], "unit": d["unit"],
            "frac": d["frac"], "traffic": None, "mean_launch_ms": d["mean_ms"],
            "all": kernels}
+    if "bytes_per_launch" in d:
+        out["bytes_per_launch"] = d["bytes_per_launch"]
+        out["flops_frac"] = d["flops_frac"]
     if window_s:
         out["headline_efficiency"] = {
             "value": ideal_s / window_s,

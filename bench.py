@@ -21,9 +21,12 @@ plain        : updates/s of the reference-compatible plain path (FP32-MFMA
                theta + torch.linalg.svd) on the same slice and gates
 fidelity     : per-update truncation fidelity of both paths against an FP64
                truncation of the same input (complex128 SVD), |dF| reported
-roofline     : the dominant single launch of the library's own (a grouped FP64
-               MFMA launch over the sub-layer: the Gram matrices or the
-               projections), achieved FP64 TFLOP/s from HIP events
+roofline     : the recorded call with the largest share of the timed region,
+               whoever wrote it -- the batched tridiagonal reduction of the
+               sub-layer's Gram matrices (one call per sub-layer) -- against
+               the bound its algorithmic work sets (HBM for the one-stage
+               reduction: the bytes its matrix-vector products must stream),
+               from HIP events; with the headline efficiency over all calls
 cpu_baseline : the CPU oracle (torch-CPU restatement of the reference update)
                on the host cores, a bounded sample of updates at the same shape
 batched      : the batched form (256 uneven MPS, RY/RZ + CZ ansatz, 8 layers,
@@ -121,6 +124,9 @@ def parse():
                     help="only time rocSOLVER's strided-batched Hermitian eigensolvers on Gram "
                          "matrices (the variants, the batch-size scan at the bench's order and "
                          "concurrent calls) and exit")
+    ap.add_argument("--hetrd-probe", type=int, default=0, metavar="B",
+                    help="only run one batched tridiagonal reduction of B Gram matrices of "
+                         "order 2 chi (the update's dominant call; for counter runs) and exit")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters for the traffic figure) and exit")
@@ -290,6 +296,8 @@ def sampler_bench(torch, a, dev):
            "roofline": {"bound": "mfma", "achieved": tf, "peak": 157.3, "unit": "TFLOP/s",
                         "frac": tf / 157.3, "flops_per_call": flops,
                         "hbm_GBps": gbs, "hbm_frac": gbs / 8000.0}}
+    out["roofline"]["traffic"], out["roofline"]["traffic_source"] = traffic_from_profile(
+        "aq_sample", flops)
     if not a.no_cpu and a.sampler_cpu_shots > 0:
         from oracle import ref_mps as R
         cores = len(os.sched_getaffinity(0))
@@ -798,6 +806,19 @@ def main():
         return
     if a.eig_probe:
         eig_probe(torch, a)
+        return
+    if a.hetrd_probe:
+        from atlas_q.eig import GramTridiag
+        n = 2 * a.chi
+        g = torch.Generator(device="cuda").manual_seed(a.seed)
+        X = torch.randn((n, n), dtype=torch.complex128, device="cuda", generator=g)
+        Gb = (X @ X.conj().T).expand(a.hetrd_probe, n, n).contiguous()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        GramTridiag(Gb)
+        torch.cuda.synchronize()
+        print(json.dumps({"hetrd_probe_batch": a.hetrd_probe, "order": n,
+                          "ms_per_call": 1e3 * (time.perf_counter() - t0)}), flush=True)
         return
     if a.copy_probe:
         src = torch.ones(a.copy_probe << 18, dtype=torch.float32, device="cuda")

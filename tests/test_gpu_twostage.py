@@ -148,3 +148,50 @@ def test_bulge_chasing_matches_lapack(n):
             assert rec < 1e-13, rec
         print(msg, flush=True)
         assert ew < 1e-13
+
+
+def test_stage_times_order_2048_batch_64():
+    """Times of both stages and of the one-stage reduction on the update's
+    shape (64 Gram matrices of order 2048; printed), with the tridiagonal
+    eigenvalues of both routes against each other (1e-13 relative)."""
+    from atlas_q.eig import GramTridiag
+    n, B = 2048, 64
+    L = lib()
+    g = torch.Generator(device=DEV).manual_seed(9)
+    X = torch.randn((n, n), dtype=torch.complex128, device=DEV, generator=g)
+    G = (X @ X.conj().T).expand(B, n, n).contiguous()
+    npan = int(L.aq_he2hb_panels(n))
+    AB = torch.empty((B, n, 2 * PB), dtype=torch.complex128, device=DEV)
+    Tall = torch.empty((B, npan, PB, PB), dtype=torch.complex128, device=DEV)
+    nws = int(L.aq_he2hb_workspace_elems(n, B))
+    ws = torch.empty(nws, dtype=torch.complex128, device=DEV)
+    nlog = int(L.aq_hb2st_log_tasks(n))
+    Vlog = torch.empty((B, nlog, PB), dtype=torch.complex128, device=DEV)
+    Tlog = torch.empty((B, nlog), dtype=torch.complex128, device=DEV)
+    D = torch.empty((B, n), dtype=torch.float64, device=DEV)
+    E = torch.empty((B, n - 1), dtype=torch.float64, device=DEV)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
+    A = G.clone()
+    torch.cuda.synchronize()
+    ev[0].record()
+    check(L.aq_he2hb_own(n, K._ptr(A), n * n, K._ptr(AB), K._ptr(Tall), K._ptr(ws), nws, B,
+                         K._stream(A)), "aq_he2hb_own")
+    ev[1].record()
+    check(L.aq_hb2st_own(n, K._ptr(AB), K._ptr(Vlog), K._ptr(Tlog), K._ptr(D), K._ptr(E), B,
+                         K._stream(A)), "aq_hb2st_own")
+    ev[2].record()
+    A2 = G.clone()
+    ev[3].record()
+    tri = GramTridiag(A2)
+    ev[4].record()
+    torch.cuda.synchronize()
+    t1, t2, t0 = (ev[0].elapsed_time(ev[1]) / B, ev[1].elapsed_time(ev[2]) / B,
+                  ev[3].elapsed_time(ev[4]) / B)
+    print(f"  per matrix: band reduction {t1:.2f} ms, bulge chasing {t2:.2f} ms, "
+          f"one-stage reduction {t0:.2f} ms", flush=True)
+    T2 = np.diag(D[0].cpu().numpy()) + np.diag(E[0].cpu().numpy(), 1) + \
+        np.diag(E[0].cpu().numpy(), -1)
+    d1, e1 = tri.D[0].cpu().numpy(), tri.E[0].cpu().numpy()
+    T1 = np.diag(d1) + np.diag(e1, 1) + np.diag(e1, -1)
+    w1, w2 = np.linalg.eigvalsh(T1), np.linalg.eigvalsh(T2)
+    assert np.abs(w1 - w2).max() < 1e-13 * np.abs(w1).max()
