@@ -4,7 +4,7 @@ For X [m, n] (complex64) and a width l = k_guess + p, p = max(16, k_guess):
   Y = X Omega                      range finder (MFMA GEMM, BF16x3 optional)
   q x { Z = X^H Q ; Y = X Z' }     optional power steps (q = POWER_STEPS),
                                    re-orthonormalised
-  Q = orth(Y)                      Householder QR in FP64 (the library's)
+  Q = orth(Y)                      shifted Cholesky-QR, twice, in FP64
   B = Q^H X        [l, n]          FP64 MFMA GEMM
   B B^H = U_B diag(lam) U_B^H      FP64 Hermitian core
   tail = ||X||_F^2 - sum(lam)      FP64: weight outside range(Q)
@@ -106,12 +106,30 @@ def certify_rank(lam: np.ndarray, tail: float, eps: float, cap: int, total: floa
     return k_min if k_min == k_max else None
 
 
-def orth(Y: torch.Tensor) -> torch.Tensor:
-    """Orthonormal basis of the columns of Y [m, l] (complex128): the thin Q
-    of the library's blocked Householder QR (``kernels.qr``), which needs no
-    conditioning assumption on Y (a sketch of a low-rank theta is itself
-    numerically rank-deficient)."""
-    return K.qr(Y)[0]
+def orth(Y: torch.Tensor, infos: list) -> torch.Tensor:
+    """Orthonormal basis of the columns of Y [m, l] (complex128) by shifted
+    Cholesky-QR followed by Cholesky-QR2 (shiftedCholeskyQR3: Fukaya,
+    Kannan, Nakatsukasa, Yamamoto, Yanagisawa, SIAM J. Sci. Comput. 42 (2020)),
+    with their shift 11 (m l + l (l + 1)) u ||Y||^2 (||Y||_F^2 as the bound).
+    The first pass keeps the span of Y and makes the next two well posed.  No
+    host round trip: the factorizations' info codes are appended to `infos`
+    and checked once by the caller.  (The library's Householder QR,
+    ``kernels.qr``, would need no shift, but its 32-column panel
+    factorisation is a chain of column steps: on the sketches here, m = 2048
+    and l ~ 100-250, three of them per update halved the low-entanglement
+    rate, 192 -> 86 updates/s; the Cholesky factor of the small l x l Gram
+    matrix costs far less.)"""
+    m, l = Y.shape
+    u = 2.0 ** -53
+    for it in range(3):
+        W = K.cgemm(Y.T, Y, conj_a=True)
+        if it == 0:
+            shift = 11.0 * (m * l + l * (l + 1)) * u * torch.diagonal(W).real.sum()
+            W = W + shift * torch.eye(l, dtype=W.dtype, device=W.device)
+        L, info = torch.linalg.cholesky_ex(W)
+        infos.append(info)
+        Y = torch.linalg.solve_triangular(L.conj().T, Y, upper=True, left=False)
+    return Y
 
 
 _TLS = __import__("threading").local()
@@ -152,22 +170,24 @@ def try_rsvd(mps, i, X, cap, budget_ok, local, cl, cr, st, k_guess):
     with RL.timed("rsvd_rangefinder", 8.0 * m * n * l,
                   "bf16_mfma" if prec == "bf16x3" else "fp32_mfma"):
         Y = K.cgemm(X, Om, prec=prec)
-    Q = orth(Y.to(torch.complex128))
+    infos = []
+    Q = orth(Y.to(torch.complex128), infos)
     for _ in range(POWER_STEPS):
         Z = K.cgemm(X.T, Q.to(torch.complex64), conj_a=True, prec=prec)
-        Zq = orth(Z.to(torch.complex128))
+        Zq = orth(Z.to(torch.complex128), infos)
         Y = K.cgemm(X, Zq.to(torch.complex64), prec=prec)
-        Q = orth(Y.to(torch.complex128))
+        Q = orth(Y.to(torch.complex128), infos)
     X64 = X.to(torch.complex128)
     with RL.timed("rsvd_project_fp64", 8.0 * l * m * n, "fp64_mfma"):
         Bc = K.cgemm(Q.T, X64, conj_a=True)                       # [l, n]
     G = K.cgemm(Bc, Bc.T, conj_b=True)                           # [l, l]
     lam, UB = eigh_desc(G)
     lam = torch.clamp(lam, min=0.0)
-    # one device-to-host transfer: captured spectrum and ||X||^2
-    host = torch.cat([lam, torch.vdot(X64.reshape(-1), X64.reshape(-1)).real.view(1)]).cpu().numpy()
-    lam_h, total = host[:-1], float(host[-1])
-    if not np.isfinite(host).all():
+    # one device-to-host transfer: captured spectrum, ||X||^2, factorization codes
+    host = torch.cat([lam, torch.vdot(X64.reshape(-1), X64.reshape(-1)).real.view(1),
+                      torch.stack(infos).to(torch.float64).abs().sum().view(1)]).cpu().numpy()
+    lam_h, total, bad = host[:-2], float(host[-2]), host[-1] != 0
+    if bad or not np.isfinite(host).all():
         st.reject("orth")
         return None, "orth"
     tail = max(total - float(lam_h.sum()), 0.0)
