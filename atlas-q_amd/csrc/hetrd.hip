@@ -583,7 +583,9 @@ extern "C" int aq_hetrd_own(int64_t n, int64_t nb, void* Av, int64_t strideA, do
   // The batch runs as up to two groups on worker streams, their column steps
   // interleaved: one group's small per-matrix step overlaps the other's
   // streaming matrix-vector launch.
-  const int ngroups = batch >= 8 ? kHetrdStreams : 1;
+  // (from two matrices on: at small batches every launch is latency-bound,
+  // and the second group's chain fills the first one's gaps)
+  const int ngroups = batch >= 2 ? kHetrdStreams : 1;
   hipStream_t ws_streams[kHetrdStreams];
   HetrdGroup g[kHetrdStreams];
   if (ngroups > 1) {

@@ -1,8 +1,11 @@
 // Top-k eigenpairs of batched Hermitian Gram cores from their tridiagonal
 // form (FP64):
 //
-//   aq_hetrd_batched       dense -> tridiagonal (rocSOLVER zhetrd, strided
-//                          batched; Householder vectors left in the matrix)
+//   aq_hetrd_batched       dense -> tridiagonal by rocSOLVER's zhetrd (strided
+//                          batched; Householder vectors left in the matrix),
+//                          the comparison point of the library's own blocked
+//                          reduction (hetrd.hip), which produces the same
+//                          storage
 //   aq_tridiag_eigvals     all or the leading eigenvalues by bisection on
 //                          Sturm counts, one thread per eigenvalue
 //   aq_tridiag_eigvecs     eigenvectors of the leading eigenvalues by one
@@ -15,7 +18,7 @@
 //   aq_hetrd_vblocks       the Householder vectors regrouped into blocks of
 //                          nb for the MFMA back-transformation, with the
 //                          blocks' compact-WY factors
-//   aq_orth_band_check     the FP64 orthogonality re-check of the computed
+//   aq_orth_band_reduce    the FP64 orthogonality re-check of the computed
 //                          vectors (Gram blocks along the diagonal)
 //
 // Layout.  The Gram matrices are row-major n x n.  rocSOLVER reads them as
@@ -408,7 +411,10 @@ extern "C" int aq_tridiag_eigvecs(int64_t n, const double* D, int64_t sD, const 
                                   int64_t kmax, void* Z, int64_t sZ, double* scratch,
                                   int64_t scratch_doubles, int64_t batch, hipStream_t stream) {
   AQ_REQUIRE(D && E && W && Z && scratch, AQ_ERR_ARG, "aq_tridiag_eigvecs: null pointer");
-  AQ_REQUIRE(n >= 1 && n <= kTriMaxN && kmax >= 1 && kmax <= n, AQ_ERR_ARG,
+  // rows k[b] .. kmax-1 are zeroed, so kmax (the row count of Z, padded by the
+  // caller) may exceed n when every k[b] <= n; without kvec every row up to
+  // kmax is computed and kmax <= n
+  AQ_REQUIRE(n >= 1 && n <= kTriMaxN && kmax >= 1 && (kvec ? true : kmax <= n), AQ_ERR_ARG,
              "aq_tridiag_eigvecs: n %lld / kmax %lld out of range", (long long)n,
              (long long)kmax);
   AQ_REQUIRE(batch >= 1 && batch <= 65535 && sZ >= kmax * n, AQ_ERR_ARG,

@@ -32,6 +32,10 @@ cpu_baseline : the CPU oracle (torch-CPU restatement of the reference update)
 batched      : the batched form (256 uneven MPS, RY/RZ + CZ ansatz, 8 layers,
                1000 shots each): updates/s of the grouped engine, shots/s of
                its graph-captured sweep, and the one-at-a-time plain rate
+circuit      : the split-precision circuit itself from |0...0> (128 qubits,
+               depth 60, chi_max 1024): updates/s per phase (growth, saturated),
+               which factorization served the updates, and growth-phase
+               truncation fidelities against FP64
 sampler      : shots/s of the fused sweep sampler on the plain-GPU circuit's
                shape (64 qubits, chi_max 512, 10,000 shots, complex64, a random
                right-canonical MPS at the circuit's bond profile), its FP32-MFMA
@@ -107,6 +111,15 @@ def parse():
     ap.add_argument("--batched-cpu-mps", type=int, default=2,
                     help="MPS of the batched population run by the oracle on the host cores")
     ap.add_argument("--no-wide", action="store_true")
+    ap.add_argument("--circuit-qubits", type=int, default=128)
+    ap.add_argument("--circuit-depth", type=int, default=60)
+    ap.add_argument("--circuit-fidelity", type=int, default=6,
+                    help="growth-phase layers whose middle gate's truncation fidelity is "
+                         "checked against the FP64 truncation")
+    ap.add_argument("--no-circuit", action="store_true",
+                    help="skip the whole split-precision circuit from |0...0>")
+    ap.add_argument("--circuit-only", action="store_true",
+                    help="only run the whole-circuit measurement and print its JSON object")
     ap.add_argument("--no-peaks", action="store_true",
                     help="skip the copy / rocBLAS GEMM reference rates")
     ap.add_argument("--wide-only", action="store_true")
@@ -214,6 +227,82 @@ def fidelity_vs_fp64(torch, A, B, gate, chi, methods):
         out[name] = {"F": F, "kept_weight": kept}
     out["fp64_kept_weight"] = float((S[:k] ** 2).sum() / (S ** 2).sum())
     return out
+
+
+def circuit_bench(torch, a, dev):
+    """The split-precision circuit itself, from |0...0>: 128 qubits, depth 60,
+    Haar SU(4) brickwork, chi_max 1024, cutoff 1e-8, complex64, learned +
+    BF16x3, every layer's gates concurrently as in the slice measurement.
+    Per layer: wall time, which factorization served each update, the largest
+    kept rank; phases split where the largest kept rank first reaches the cap.
+    Between layers (outside the timing) the touched sites are rescaled to unit
+    norm, as in the slice measurement (the reference's update does not
+    renormalise, and ~9 % discarded weight per saturated update would take the
+    complex64 state's scale to underflow within a few hundred layers' worth of
+    updates).  In the growth phase the middle gate of every few layers is
+    checked against the FP64 truncation of the same theta."""
+    from atlas_q.adaptive_mps import AdaptiveMPS, DTypePolicy, _gate4
+    from atlas_q.circuits import brickwork
+    n, depth, chi = a.circuit_qubits, a.circuit_depth, a.chi
+    ops = brickwork(n, depth, a.seed + 31)
+    m = AdaptiveMPS(n, bond_dim=1, eps_bond=1e-8, chi_max_per_bond=chi,
+                    dtype_policy=DTypePolicy(default=torch.complex64), device="cuda",
+                    svd_method=a.method, theta_precision=a.theta_precision)
+    layers = sublayers(ops)
+    fid_every = max(1, 12 // max(1, a.circuit_fidelity))
+    per, fids = [], []
+    counts0 = None
+    for li, run in enumerate(layers):
+        st = getattr(m, "factor_stats", None)
+        before = dict(st.counts) if st is not None else {}
+        sample = None
+        if len(fids) < a.circuit_fidelity and li % fid_every == fid_every - 1:
+            b, U = run[len(run) // 2]
+            sample = (b, U, m.tensors[b].clone(), m.tensors[b + 1].clone())
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        m.apply_two_site_gates(run, workers=a.workers)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        st = getattr(m, "factor_stats", None)
+        after = dict(st.counts) if st is not None else {}
+        delta = {k: after.get(k, 0) - before.get(k, 0) for k in after}
+        kmax = max(int(m.tensors[b].shape[2]) for b, _ in run)
+        per.append({"layer": li, "updates": len(run), "s": dt, "factor_counts": delta,
+                    "max_kept_rank": kmax})
+        if sample is not None:
+            b, U, A0, B0 = sample
+            f = fidelity_vs_fp64(torch, A0, B0, _gate4(U), chi,
+                                 {"fast": (m.tensors[b], m.tensors[b + 1])})
+            fids.append({"layer": li, "bond": b, "kept_rank": int(m.tensors[b].shape[2]),
+                         "F": f["fast"]["F"], "fp64_kept_weight": f["fp64_kept_weight"]})
+        for b, _ in run:
+            for q in (b, b + 1):
+                m.tensors[q] = m.tensors[q] / m.tensors[q].norm()
+    sat = next((p["layer"] for p in per if p["max_kept_rank"] >= chi), len(per))
+
+    def phase(rows, name):
+        if not rows:
+            return None
+        cnt = {}
+        for r in rows:
+            for k, v in r["factor_counts"].items():
+                cnt[k] = cnt.get(k, 0) + v
+        ups = sum(r["updates"] for r in rows)
+        t = sum(r["s"] for r in rows)
+        return {"phase": name, "layers": f"{rows[0]['layer']}-{rows[-1]['layer']}",
+                "updates": ups, "s": t, "updates_per_s": ups / t, "factor_counts": cnt}
+    total_u = sum(r["updates"] for r in per)
+    total_t = sum(r["s"] for r in per)
+    return {"workload": f"{n}-qubit Haar SU(4) brickwork, depth {depth}, chi_max {chi}, cutoff "
+                        f"1e-8, complex64, from |0...0>, {a.method} + {a.theta_precision}",
+            "updates": total_u, "s": total_t, "updates_per_s": total_u / total_t,
+            "phases": [p for p in (phase(per[:sat], "growth (largest kept rank below the cap)"),
+                                   phase(per[sat:], "saturated")) if p],
+            "per_layer_updates_per_s": [round(r["updates"] / r["s"], 1) for r in per],
+            "per_layer_max_kept_rank": [r["max_kept_rank"] for r in per],
+            "fidelity_growth_samples": fids,
+            "min_fidelity": min((f["F"] for f in fids), default=None)}
 
 
 def low_entanglement(torch, a, dev):
@@ -840,6 +929,9 @@ def main():
     if a.wide_only:
         print(json.dumps({"learned_rank_form": wide_bench(torch, a, dev)}), flush=True)
         return
+    if a.circuit_only:
+        print(json.dumps({"circuit": circuit_bench(torch, a, dev)}), flush=True)
+        return
     chi, sites = a.chi, a.sites
     base = slice_state(torch, sites, chi, a.seed, dev)
     total = a.warmup + a.steps
@@ -938,6 +1030,8 @@ def main():
     log("batched done")
     wide = None if a.no_wide else wide_bench(torch, a, dev)
     log("learned-rank form (chi 2048) done")
+    circ = None if a.no_circuit else circuit_bench(torch, a, dev)
+    log("whole circuit done")
 
     peaks = None if a.no_peaks else measured_peaks(torch, dev)
     log("measured peaks done")
@@ -969,6 +1063,7 @@ def main():
         "factor_counts": fast.factor_stats.counts if hasattr(fast, "factor_stats") else None,
         "factor_rejects": fast.factor_stats.rejects if hasattr(fast, "factor_stats") else None,
         "low_entanglement": low,
+        "circuit": circ,
         "roofline": roof,
         "eigensolver": eig,
         "cpu_baseline": cpu,

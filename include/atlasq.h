@@ -226,7 +226,9 @@ int aq_bgemm_lower(int64_t M, int64_t K, double alpha, const void* A, int64_t a_
  * aq_tridiag_eigvecs: for j < kvec[b] (all j < kmax when kvec is NULL), row
  * j of Zt (complex128 [batch, >= kmax, n], matrix stride sZ) = the unit
  * eigenvector of W[b, j] (twisted inverse iteration; rows kvec[b] .. kmax-1
- * zeroed); scratch: aq_tridiag_eigvecs_scratch_doubles() doubles.
+ * zeroed, so with kvec the padded row count kmax may exceed n as long as
+ * every kvec[b] <= n); scratch: aq_tridiag_eigvecs_scratch_doubles()
+ * doubles.
  * aq_hetrd_vblocks / aq_hetrd_tfactors: the vectors v_i in blocks of nb <=
  * 64 (V [batch, nblk, nb, n], nblk = ceil((n-1)/nb), zero rows past i = n-2)
  * and each block's forward compact-WY factor T' (nb x nb, from the block's
@@ -239,6 +241,19 @@ int aq_bgemm_lower(int64_t M, int64_t K, double alpha, const void* A, int64_t a_
  * transpose of rows 64t .. 64t+127), restricted to indices < kvec[b]. */
 int aq_hetrd_batched(int64_t n, void* A, int64_t strideA, double* D, double* E, void* tau,
                      int64_t batch, hipStream_t stream);
+int aq_tridiag_eigvals(int64_t n, const double* D, int64_t sD, const double* E, int64_t sE,
+                       double* W, int64_t sW, int64_t nval, int64_t batch, hipStream_t stream);
+int64_t aq_tridiag_eigvecs_scratch_doubles(int64_t n, int64_t kmax, int64_t batch);
+int aq_tridiag_eigvecs(int64_t n, const double* D, int64_t sD, const double* E, int64_t sE,
+                       const double* W, int64_t sW, const int32_t* kvec, int64_t kmax, void* Zt,
+                       int64_t sZ, double* scratch, int64_t scratch_doubles, int64_t batch,
+                       hipStream_t stream);
+int aq_hetrd_vblocks(int64_t n, int64_t nb, const void* A, int64_t strideA, void* V,
+                     int64_t batch, hipStream_t stream);
+int aq_hetrd_tfactors(int64_t n, int64_t nb, const void* tau, int64_t stau, const void* Gv,
+                      void* Tf, int64_t batch, hipStream_t stream);
+int aq_orth_band_reduce(const void* Q, int64_t ntile, const int32_t* kvec, int64_t kmax,
+                        double* err, int64_t batch, hipStream_t stream);
 /* The same reduction by the library's own kernels (same outputs and storage
  * as aq_hetrd_batched; LAPACK's blocked zhetrd/zlatrd with uplo "U"): panels
  * of nb <= 32 columns; per column one batched streaming Hermitian
@@ -246,7 +261,7 @@ int aq_hetrd_batched(int64_t n, void* A, int64_t strideA, double* D, double* E, 
  * quadratic form and the panel dot products), one wide kernel forming the W
  * column and updating the next column, and one per-matrix reflector kernel;
  * per panel a rank-2nb update of the lower triangle on MFMA
- * (aq_bgemm_lower).  Batches of 8 or more run as two interleaved groups on
+ * (aq_bgemm_lower).  Batches of 2 or more run as two interleaved groups on
  * the library's worker streams, joined to `stream` by events.
  * ws: aq_hetrd_workspace_elems(n, nb, batch) complex128. */
 int64_t aq_hetrd_workspace_elems(int64_t n, int64_t nb, int64_t batch);
@@ -293,19 +308,6 @@ int aq_geqrf(int64_t m, int64_t n, void* A, int64_t strideA, void* Tall, void* w
              int64_t ws_elems, int64_t batch, hipStream_t stream);
 int aq_ungqr(int64_t m, int64_t n, const void* A, int64_t strideA, const void* Tall, void* Q,
              int64_t strideQ, void* ws, int64_t ws_elems, int64_t batch, hipStream_t stream);
-int aq_tridiag_eigvals(int64_t n, const double* D, int64_t sD, const double* E, int64_t sE,
-                       double* W, int64_t sW, int64_t nval, int64_t batch, hipStream_t stream);
-int64_t aq_tridiag_eigvecs_scratch_doubles(int64_t n, int64_t kmax, int64_t batch);
-int aq_tridiag_eigvecs(int64_t n, const double* D, int64_t sD, const double* E, int64_t sE,
-                       const double* W, int64_t sW, const int32_t* kvec, int64_t kmax, void* Zt,
-                       int64_t sZ, double* scratch, int64_t scratch_doubles, int64_t batch,
-                       hipStream_t stream);
-int aq_hetrd_vblocks(int64_t n, int64_t nb, const void* A, int64_t strideA, void* V,
-                     int64_t batch, hipStream_t stream);
-int aq_hetrd_tfactors(int64_t n, int64_t nb, const void* tau, int64_t stau, const void* Gv,
-                      void* Tf, int64_t batch, hipStream_t stream);
-int aq_orth_band_reduce(const void* Q, int64_t ntile, const int32_t* kvec, int64_t kmax,
-                        double* err, int64_t batch, hipStream_t stream);
 
 /* Rank-predictor MLP 32 -> 128 -> 128 -> 1 with GELU and BF16 weights (no
  * reference counterpart for pre-SVD prediction; the reference's predictors,

@@ -194,3 +194,12 @@ def test_own_reduction_time_against_rocsolver():
         e.record()
         torch.cuda.synchronize()
         print(f"  {red}: {s.elapsed_time(e) / 16:.2f} ms per matrix (batch 16, n 2048)", flush=True)
+
+
+def test_tridiag_kept_rank_near_order():
+    """Kept ranks whose 64-row padding exceeds the order (k = 96 and 100 of
+    100; 500 of 506, as in a growing circuit's Gram cores): the padded rows
+    stay zero and the kept vectors are exact."""
+    rng = np.random.default_rng(12)
+    _check([_gram(rng, 100, 100), _gram(rng, 100, 100)], [96, 100], "n=100 near-full")
+    _check([_gram(rng, 506, 506)], [500], "n=506 near-full")
