@@ -10,9 +10,14 @@ dimensions are tracked on the host.  A circuit layer is executed as
   two-qubit sub-layer : grouped theta (MFMA, gate in the epilogue)
                         -> FP64 Gram of the smaller side (grouped GEMM)
                         -> eigen-decomposition per size bucket (Jacobi kernel
-                           for orders <= 64, rocSOLVER batched above)
+                           for orders <= 64; above, the library's tridiagonal
+                           eigensolver: reduction, all eigenvalues by
+                           bisection)
                         -> batched rank rule (one launch, one 48-byte-per-gate
-                           device->host copy: the only sync of the sub-layer)
+                           device->host copy)
+                        -> eigenvectors of the kept ranks only (tridiagonal
+                           buckets: inverse iteration, MFMA back-transformation,
+                           FP64 orthogonality re-check: a second small copy)
                         -> grouped projection GEMM + grouped rebuild into the
                            slots
 
@@ -36,7 +41,7 @@ import torch
 from . import _ffi
 from ._ffi import (COPY_DESC, GEMM_DESC, SAMPLE_DESC, SITE1Q_DESC, RankResult, check, lib)
 from .autotune import Autotuner
-from .eig import JACOBI_MAX, finite_rows, heevd_batched
+from .eig import JACOBI_MAX, ORTH_TOL, GramTridiag, finite_rows, heevd_batched
 from .linalg_robust import robust_svd
 from .rank_mlp import heev_small
 
@@ -157,6 +162,19 @@ def _condition_bucket(T: torch.Tensor, orders: np.ndarray, jitter: torch.Tensor)
         T.add_(jitter.unsqueeze(0) * (1e-13 * d).unsqueeze(-1).to(T.dtype))
 
 
+def _pad_bucket(T: torch.Tensor, orders: np.ndarray):
+    """Prepare a size bucket for the tridiagonal eigensolver: the padding
+    diagonal of matrix b (rows/columns >= orders[b], zero elsewhere) gets the
+    distinct negative values -d_b (1 + i / size), d_b the Gram's largest
+    diagonal entry.  The padded matrix is the Gram block plus a diagonal
+    block, so the reduction leaves the padding columns alone (their
+    reflectors are the identity), the tridiagonal form splits exactly at the
+    block boundary, and every padding eigenvalue sorts below the Gram's (which
+    are >= 0) and apart from its null space.  No perturbation is added: the
+    solver treats exactly repeated eigenvalues by splitting the tridiagonal."""
+    _condition_bucket(T, orders, None)
+
+
 @contextlib.contextmanager
 def _gc_paused():
     """Pause Python's cyclic garbage collector (restored on exit).  The
@@ -184,12 +202,19 @@ class BatchedMPS:
     """Population of MPS (n_j qubits, cap chi_j) starting in |0...0>."""
 
     def __init__(self, specs: Sequence[Tuple[int, int]], eps_bond: float = 1e-8,
-                 device="cuda", autotuner: Autotuner = None, eig_workers: int = 3):
+                 device="cuda", autotuner: Autotuner = None, eig_workers: int = 3,
+                 eig_solver: str = "tridiag"):
         self.dev = torch.device(device)
-        # host threads (one HIP stream each) for the rocSOLVER size buckets of a
-        # sub-layer: each call is a chain of small per-column launches, so
-        # buckets solved concurrently overlap their launch latencies
+        # host threads (one HIP stream each) for the size buckets above the
+        # Jacobi limit: each bucket's solve is a chain of small per-column
+        # launches, so buckets solved concurrently overlap their latencies
         self.eig_workers = int(eig_workers)
+        # orders above the Jacobi limit: "tridiag" (the library's reduction,
+        # bisection and top-k inverse iteration) or "heevd" (rocSOLVER's
+        # divide-and-conquer, kept as the comparison point)
+        if eig_solver not in ("tridiag", "heevd"):
+            raise ValueError(f"unknown eig_solver {eig_solver!r}")
+        self.eig_solver = eig_solver
         if self.dev.type != "cuda":
             raise RuntimeError("BatchedMPS runs on a HIP device only")
         _ffi.check_layouts()
@@ -272,16 +297,27 @@ class BatchedMPS:
 
     # ------------------------------------------------------------ 2q layer
     def _grouped_gemm(self, dtype_code, descs: np.ndarray, layout: int, key: str):
+        self._launch_grouped(self._prep_grouped(dtype_code, descs, layout, key))
+
+    def _prep_grouped(self, dtype_code, descs: np.ndarray, layout: int, key: str):
+        """Descriptors and tile offsets of one grouped GEMM, uploaded (queued
+        on the stream); ``_launch_grouped`` launches it."""
         if len(descs) == 0:
-            return
+            return None
         paired = descs["kind"] == _ffi.AQ_GROUPED_THETA
         tn = np.where(paired, _cdiv(descs["N"] // 2, 64), _cdiv(descs["N"], 128))
         tiles = _cdiv(descs["M"], 64) * tn
         pre, total = _prefix(tiles)
         dd, pd = _upload(descs, self.dev), _h2d(pre, self.dev)
+        return dtype_code, dd, pd, len(descs), total, layout, key
+
+    def _launch_grouped(self, prep):
+        if prep is None:
+            return
+        dtype_code, dd, pd, count, total, layout, key = prep
 
         def run(g):
-            check(lib().aq_grouped_gemm(dtype_code, _vp(dd), _vp(pd), len(descs), total, layout,
+            check(lib().aq_grouped_gemm(dtype_code, _vp(dd), _vp(pd), count, total, layout,
                                         g, _stream()), "aq_grouped_gemm")
         run(self.tuner.grid(key, total, run))
 
@@ -355,52 +391,43 @@ class BatchedMPS:
         self._grouped_gemm(_ffi.AQ_C128, gram[left], 1 | 4, "gram")
         self._grouped_gemm(_ffi.AQ_C128, gram[~left], 2 | 4, "gram")
         # ---- eigen-decompositions per bucket (descending); success flags stay
-        # on the device and come back with the rank results (one sync).  The
-        # rocSOLVER buckets run on worker threads with their own streams
-        # while the Jacobi buckets (one launch each) run on this one.
+        # on the device and come back with the rank results.  Buckets above
+        # the Jacobi limit run on worker threads with their own streams while
+        # the Jacobi buckets (one launch each) run on this one.  The
+        # tridiagonal solver gives the eigenvalues first; the vectors of the
+        # kept ranks follow the rank rule.
         big = [sz for sz in buckets if sz > JACOBI_MAX]
-        for sz in big:
-            if sz not in self._jitter:
-                self._jitter[sz] = _jitter_matrix(sz, self.dev)
+        tri_mode = self.eig_solver == "tridiag"
+        if not tri_mode:
+            for sz in big:
+                if sz not in self._jitter:
+                    self._jitter[sz] = _jitter_matrix(sz, self.dev)
 
         def solve_big(sz):
             idx, T = buckets[sz]
+            if tri_mode:
+                _pad_bucket(T, s[idx])
+                tri = GramTridiag(T)
+                W = tri.eigenvalues()
+                return W, tri, torch.ones(len(idx), dtype=torch.bool, device=self.dev)
             _condition_bucket(T, s[idx], self._jitter[sz])
             return heevd_batched(T, inplace=True, return_ok="device")
 
         main = torch.cuda.current_stream(self.dev)
-        futs = {}
-        if self.eig_workers > 1 and len(big) > 1:
-            from .adaptive_mps import _thread_stream, _worker_pool
-            ready = main.record_event()
-
-            def job(sz):
-                st = _thread_stream(self.dev)
-                with torch.cuda.stream(st):
-                    st.wait_event(ready)
-                    out = solve_big(sz)
-                    done = st.record_event()
-                return out, done
-
-            pool = _worker_pool(self.eig_workers)
-            futs = {sz: pool.submit(job, sz) for sz in big}
         res = {}
         for sz, (idx, T) in buckets.items():
             if sz <= JACOBI_MAX:
                 W, V = heev_small(T, orders=_h2d(s[idx].astype(np.int64), self.dev))
                 res[sz] = (W, V, finite_rows(W, V))
-            elif not futs:
-                res[sz] = solve_big(sz)
-        for sz, f in futs.items():
-            out, done = f.result()
-            main.wait_event(done)
-            for t in out:                 # produced on a worker stream, read on this one
-                t.record_stream(main)
-            res[sz] = out
-        W_all, V_all, oks = {}, {}, []
+        res.update(self._on_workers(big, solve_big, main))
+        W_all, V_all, oks, tris = {}, {}, [], {}
         for sz in buckets:
             W, V, ok = res[sz]
-            W_all[sz], V_all[sz] = W.contiguous(), V.contiguous()
+            W_all[sz] = W.contiguous()
+            if isinstance(V, GramTridiag):
+                tris[sz] = V
+            else:
+                V_all[sz] = V.contiguous()
             oks.append(ok)
         soff, _ = _prefix(s)
         rr, okh = self._rank_all(buckets, W_all, s, soff, jj, torch.cat(oks))
@@ -416,11 +443,39 @@ class BatchedMPS:
             rr, _ = self._rank_all(buckets, W_all, s, soff, jj, None)
         Sf64 = self._last_sv
         k = rr["k"].astype(np.int64)
-        # ---- projections (FP64) and rebuild into the slots
+        # eigenvector accessor per gate: V[r, c] = (conj) base[r vs0 + c vs1]
+        # (component r of the eigenvector of the c-th largest eigenvalue)
         vptr = np.zeros(G, np.int64)
+        vs0 = np.zeros(G, np.int64)
+        vs1 = np.zeros(G, np.int64)
+        vcj = np.zeros(G, np.int64)
+        for sz, (idx, T) in buckets.items():
+            if sz in V_all:            # columns of a row-major sz x sz block
+                vptr[idx] = V_all[sz].data_ptr() + 16 * sz * sz * np.arange(len(idx))
+                vs0[idx], vs1[idx] = sz, 1
+        ok_host = ok_ev = None
+        if tris:
+            # on this stream: the solvers' buffers are marked as used here.
+            # The re-check flags come back through pinned memory; the
+            # projection and copy descriptors are built while the vectors
+            # are computed, and the flags are waited for just before launch.
+            vec = {sz: tris[sz].vectors(k[buckets[sz][0]].tolist()) for sz in tris}
+            errs = torch.cat([vec[sz][1] for sz in tris])
+            okd = (torch.isfinite(errs) & (errs <= ORTH_TOL)).to(torch.uint8)
+            ok_host = torch.empty(okd.shape, dtype=torch.uint8, pin_memory=True)
+            ok_host.copy_(okd, non_blocking=True)
+            ok_ev = torch.cuda.current_stream(self.dev).record_event()
+            for sz in tris:
+                idx = buckets[sz][0]
+                Et = vec[sz][0]
+                kpad = Et.shape[1]
+                V_all[sz] = Et
+                # row j of Et[b] is the conjugate transpose of eigenvector j
+                vptr[idx] = Et.data_ptr() + 16 * kpad * sz * np.arange(len(idx))
+                vs0[idx], vs1[idx], vcj[idx] = 1, sz, 1
+        # ---- projections (FP64) and rebuild into the slots
         sptr = np.zeros(G, np.int64)
         for sz, (idx, T) in buckets.items():
-            vptr[idx] = V_all[sz].data_ptr() + 16 * sz * sz * np.arange(len(idx))
             sptr[idx] = Sf64[sz].data_ptr() + 8 * sz * np.arange(len(idx))
         inv = {sz: torch.where(v > 0, 1.0 / v, torch.zeros_like(v)) for sz, v in Sf64.items()}
         iptr = np.zeros(G, np.int64)
@@ -432,22 +487,23 @@ class BatchedMPS:
         pj = np.zeros(G, GEMM_DESC)
         pj["kind"] = _ffi.AQ_GROUPED_GEMM
         pj["C"] = P.data_ptr() + 16 * poff
-        # left: Vh_k = U_k^H X : A(i,r) = conj V[r,i] (V row-major, ld = bucket size)
+        # left: Vh_k = U_k^H X : A(i,r) = conj V[r,i]
         pj["A"] = np.where(left, vptr, x64)
         pj["M"] = np.where(left, k, m)
         pj["N"] = np.where(left, n, k)
         pj["K"] = np.where(left, m, n)
-        pj["a_s0"] = np.where(left, 1, n)
-        pj["a_s1"] = np.where(left, gld, 1)
-        pj["conj_a"] = np.where(left, 1, 0)
+        pj["a_s0"] = np.where(left, vs1, n)
+        pj["a_s1"] = np.where(left, vs0, 1)
+        pj["conj_a"] = np.where(left, 1 - vcj, 0)
         # right: U_k S_k = X V_k : B(r,c) = V[r,c]
         pj["B"] = np.where(left, x64, vptr)
-        pj["b_s0"] = np.where(left, n, gld)
-        pj["b_s1"] = 1
+        pj["b_s0"] = np.where(left, n, vs0)
+        pj["b_s1"] = np.where(left, 1, vs1)
+        pj["conj_b"] = np.where(left, 0, vcj)
         pj["c_s0"] = np.where(left, n, k)
         pj["c_s1"] = 1
-        self._grouped_gemm(_ffi.AQ_C128, pj[left], 2 | 4, "proj")
-        self._grouped_gemm(_ffi.AQ_C128, pj[~left], 3 | 4, "proj")
+        proj = (self._prep_grouped(_ffi.AQ_C128, pj[left], 2 | 4, "proj"),
+                self._prep_grouped(_ffi.AQ_C128, pj[~left], 3 | 4, "proj"))
         # new shapes: site b -> [dl, 2, k], site b+1 -> [k, 2, dr]
         cp = np.zeros(2 * G, COPY_DESC)
         A_, B_ = cp[:G], cp[G:]
@@ -456,19 +512,34 @@ class BatchedMPS:
         # left: A' = V[:, :k] * S (col scale), B' = P * (1/S) (row scale)
         A_["src"] = np.where(left, vptr, P.data_ptr() + 16 * poff)
         A_["rows"], A_["cols"] = m, k
-        A_["s0"] = np.where(left, gld, k)
-        A_["s1"] = 1
+        A_["s0"] = np.where(left, vs0, k)
+        A_["s1"] = np.where(left, vs1, 1)
+        A_["conj"] = np.where(left, vcj, 0)
         A_["col_scale"] = np.where(left, sptr, 0)
         B_["src"] = np.where(left, P.data_ptr() + 16 * poff, vptr)
         B_["rows"], B_["cols"] = k, n
         # right: B' = V_k^H : (r,c) = conj V[c, r]
-        B_["s0"] = np.where(left, n, 1)
-        B_["s1"] = np.where(left, 1, gld)
-        B_["conj"] = np.where(left, 0, 1)
+        B_["s0"] = np.where(left, n, vs1)
+        B_["s1"] = np.where(left, 1, vs0)
+        B_["conj"] = np.where(left, 0, 1 - vcj)
         B_["row_scale"] = np.where(left, iptr, 0)
         cnt = cp["rows"] * cp["cols"]
         pre, total = _prefix(cnt)
         dd, pd = _upload(cp, self.dev), _h2d(pre, self.dev)
+        if tris:
+            ok_ev.synchronize()
+            good = ok_host.numpy().astype(bool)
+            at = 0
+            for sz in tris:
+                idx = buckets[sz][0]
+                bad = np.flatnonzero(~good[at: at + len(idx)]).tolist()
+                at += len(idx)
+                if bad:
+                    # vectors that failed the re-check: from their theta's SVD
+                    self.eig_fallbacks += len(bad)
+                    self._svd_fill_rows(V_all[sz], bad, idx, k, X64, xoff, m, n, left)
+        for prep in proj:
+            self._launch_grouped(prep)
 
         def runc(g):
             check(lib().aq_grouped_copy(_vp(dd), _vp(pd), len(cp), total, g, _stream()),
@@ -481,7 +552,8 @@ class BatchedMPS:
         self.logs["eps_local"].extend(rr["eps_local"].tolist())
         self.logs["entropy"].extend(rr["entropy"].tolist())
         # keep the temporaries alive until the stream has consumed them
-        self._keep = (X, X64, buckets, W_all, V_all, self._keep_rank, Sf64, inv, P, dd, pd)
+        self._keep = (X, X64, buckets, W_all, V_all, tris, self._keep_rank, Sf64, inv, P,
+                      dd, pd)
 
     def _rank_all(self, buckets, W_all, s, soff, jj, ok_dev):
         """Batched rank rule on the float32 singular values of every bucket;
@@ -516,6 +588,52 @@ class BatchedMPS:
         self._last_sv = Sf64
         self._keep_rank = (Sflat, sp, md, cd, buf)
         return rr, (host[nres:] != 0 if nok else None)
+
+    def _on_workers(self, sizes, fn, main):
+        """{sz: fn(sz)} for the buckets ``sizes``, each on a worker thread's own
+        stream (waiting for ``main``'s work so far) when there are several,
+        ``main`` waiting for all of them; else on ``main``."""
+        if not sizes:
+            return {}
+        if self.eig_workers <= 1 or len(sizes) == 1:
+            return {sz: fn(sz) for sz in sizes}
+        from .adaptive_mps import _thread_stream, _worker_pool
+        ready = main.record_event()
+
+        def job(sz):
+            st = _thread_stream(self.dev)
+            with torch.cuda.stream(st):
+                st.wait_event(ready)
+                out = fn(sz)
+                done = st.record_event()
+            return out, done
+
+        pool = _worker_pool(self.eig_workers)
+        futs = {sz: pool.submit(job, sz) for sz in sizes}
+        out = {}
+        for sz, f in futs.items():
+            r, done = f.result()
+            main.wait_event(done)
+            for t in r:                   # produced on a worker stream, read on this one
+                if isinstance(t, (torch.Tensor, GramTridiag)):
+                    t.record_stream(main)
+            out[sz] = r
+        return out
+
+    @staticmethod
+    def _svd_fill_rows(Et, bad, idx, k, X64, xoff, m, n, left):
+        """Rows of a tridiagonal bucket whose vectors failed the orthogonality
+        re-check, from the SVD of each one's own theta: Et[b, j, :] is the
+        conjugate transpose of the j-th left (X X^H) or right (X^H X)
+        singular vector, j < k; the rest of the row block is zero."""
+        for b in bad:
+            g = int(idx[b])
+            mg, ng, kg = int(m[g]), int(n[g]), int(k[g])
+            X = X64[int(xoff[g]): int(xoff[g]) + mg * ng].view(mg, ng)
+            U, S, Vh, _ = robust_svd(X)
+            F = U if left[g] else Vh.conj().transpose(0, 1)
+            Et[b].zero_()
+            Et[b, :kg, :F.shape[0]] = F[:, :kg].conj().transpose(0, 1)
 
     @staticmethod
     def _svd_fill(W, V, bad, idx, X64, xoff, m, n, left):

@@ -1,22 +1,24 @@
-"""Hermitian eigen-decomposition dispatch for the truncated-SVD cores.
+"""Hermitian eigen-decomposition of the truncated-SVD cores.
 
-Orders up to 64 run in the library's batched FP64 Jacobi kernel
-(``aq_heev_small``: one work-group, matrix in LDS, no launch-latency chain);
-larger orders use rocSOLVER's divide-and-conquer, either through
-``torch.linalg.eigh`` or, for a batch of cores (all Gram matrices of a
-brickwork sub-layer), through the library's strided-batched
-``aq_heevd_batched``, which pays rocSOLVER's kernel chain once per batch.
-Both return eigenvalues in descending order with matching eigenvector columns.
+The fast paths run on the library's kernels: orders up to 64 in the batched
+FP64 Jacobi kernel (``aq_heev_small``: one work-group, matrix in LDS);
+larger orders through the tridiagonal form (``GramTridiag``: the library's
+blocked reduction, all eigenvalues by bisection, eigenvectors of the kept
+ones only by inverse iteration, back-transformed on MFMA and re-checked for
+orthogonality in FP64).  ``TopEig`` serves one matrix that way: eigenvalues
+first, the rank rule on them, then the vectors of the kept rank.
 
-rocSOLVER can report non-convergence on strongly clustered spectra (many
-eigenvalues at zero, as in the Gram matrix of a rank-deficient theta), and
-on such batches it has also returned non-finite eigenvectors with a zero
-info code; both count as a failure here.  As the reference's SVD cascade
-retries with a 1e-12 jitter of the input (linalg_robust.py:31-39),
-``eigh_desc`` retries on G + E, E a random Hermitian matrix of norm ~1e-13,
-then ~1e-11, times ||G||_max, which separates the cluster and moves every
-eigenvalue by at most ||E||_2; ``heevd_batched`` reports the failed matrices
-to its caller.
+rocSOLVER's divide-and-conquer remains as the comparison point of the
+batched form (``heevd_batched``) and as the fallback of a matrix whose
+vectors fail the orthogonality re-check (``eigh_desc``).  It can report
+non-convergence on strongly clustered spectra (many eigenvalues at zero, as
+in the Gram matrix of a rank-deficient theta), and on such batches it has
+also returned non-finite eigenvectors with a zero info code; both count as a
+failure here.  As the reference's SVD cascade retries with a 1e-12 jitter of
+the input (linalg_robust.py:31-39), ``eigh_desc`` retries on G + E, E a
+random Hermitian matrix of norm ~1e-13, then ~1e-11, times ||G||_max, which
+separates the cluster and moves every eigenvalue by at most ||E||_2;
+``heevd_batched`` reports the failed matrices to its caller.
 """
 from __future__ import annotations
 
@@ -158,6 +160,13 @@ class GramTridiag:
             raise ValueError(f"unknown tridiagonal reduction {reduction!r}")
         self.W = None
 
+    def record_stream(self, stream) -> None:
+        """Mark the solver's device buffers as used on ``stream`` (for a solver
+        built on one stream and continued on another)."""
+        for t in (self.G, self.D, self.E, self.tau, getattr(self, "_ws", None), self.W):
+            if t is not None:
+                t.record_stream(stream)
+
     def eigenvalues(self) -> torch.Tensor:
         """All eigenvalues, descending [B, n] (float64)."""
         from . import kernels as K
@@ -229,3 +238,41 @@ class GramTridiag:
               "aq_orth_band_reduce")
         self._keep = (flat, kv, Q)
         return Et, err
+
+
+class TopEig:
+    """Eigenvalues (descending, ``values``) of one Hermitian complex128 matrix
+    G [n, n] and, on request, the eigenvectors of its k largest
+    (``vectors(k)``: columns [n, k]).  Orders <= 64: the Jacobi kernel (all
+    pairs in one launch).  Larger: the tridiagonal path (reduction and
+    bisection now, inverse iteration and back-transformation for the k
+    vectors later); vectors that fail the FP64 orthogonality re-check are
+    taken from ``eigh_desc`` instead (``fallbacks`` counts them)."""
+
+    fallbacks = 0
+
+    def __init__(self, G: torch.Tensor):
+        self.G = G
+        n = G.shape[-1]
+        if n <= JACOBI_MAX:
+            from .rank_mlp import heev_small
+            self.values, self._V = heev_small(G)
+            self._tri = None
+        else:
+            Gb = G.to(torch.complex128).contiguous().clone().unsqueeze(0)
+            self._tri = GramTridiag(Gb)
+            self.values = self._tri.eigenvalues()[0]
+            self._V = None
+
+    def vectors(self, k: int) -> torch.Tensor:
+        k = int(k)
+        if self._V is not None:
+            return self._V[:, :k]
+        Et, err = self._tri.vectors([k])
+        e = float(err[0])
+        if not (e <= ORTH_TOL):
+            TopEig.fallbacks += 1
+            _, V = eigh_desc(self.G)
+            self._V = V
+            return V[:, :k]
+        return Et[0, :k, :].transpose(0, 1).conj()

@@ -30,7 +30,7 @@ import torch
 
 from . import kernels as K
 from . import roofline as RL
-from .eig import eigh_desc
+from .eig import TopEig
 from .linalg_robust import robust_svd
 from .truncation import choose_rank_from_sigma
 
@@ -95,17 +95,10 @@ def _gram_matrix(X: torch.Tensor, out: torch.Tensor = None):
         return K.cgemm(X64.T, X64, conj_a=True, out=out), False, X64
 
 
-def _gram_factors(X: torch.Tensor):
-    """Exact leading factors of X via the FP64 Gram matrix of the smaller side.
-    Returns (U [m, r] c128, S [r] f64 descending, Vh [r, n] c128, X64)."""
-    G, left, X64 = _gram_matrix(X)
-    lam, V = eigh_desc(G)
-    S = torch.sqrt(torch.clamp(lam, min=0.0))
-    return (V, S, None, X64) if left else (None, S, V, X64)
-
-
-def _finish_from_gram(mps, X, U, S, Vr, X64, k_dtype, cap, budget_ok, cl, cr, eps):
-    """Rank rule on S (in the policy's real dtype) and the two cores."""
+def _finish_from_gram(mps, X, vec, left, S, X64, k_dtype, cap, budget_ok, cl, cr, eps):
+    """Rank rule on S (in the policy's real dtype) and the two cores; vec(k)
+    gives the Gram matrix's eigenvectors of the k largest eigenvalues as
+    columns (left singular vectors of X when ``left``, else right ones)."""
     S_rule = S.to(torch.float32) if k_dtype == torch.complex64 else S
     k, eps_loc, entropy, condS = choose_rank_from_sigma(S_rule, eps, cap, budget_ok)
     local = k_dtype
@@ -114,13 +107,13 @@ def _finish_from_gram(mps, X, U, S, Vr, X64, k_dtype, cap, budget_ok, cl, cr, ep
         local = torch.complex128
     Sk = S[:k]
     inv = torch.where(Sk > 0, 1.0 / Sk, torch.zeros_like(Sk))
-    if U is not None:
-        Uk = U[:, :k]
+    if left:
+        Uk = vec(k)
         with RL.timed("proj_fp64_cgemm", 8.0 * k * X64.shape[0] * X64.shape[1], "fp64_mfma"):
             Vh = K.cgemm(Uk.T, X64, conj_a=True)             # U_k^H X  [k, n]
         Vh = Vh * inv[:, None]
     else:
-        Vk = Vr[:, :k]
+        Vk = vec(k)
         Uk = K.cgemm(X64, Vk) * inv[None, :]                   # X V_k S^-1 [m, k]
         Vh = Vk.T.conj()
     Uk = Uk.to(local)
@@ -443,19 +436,19 @@ def _gram_group_tridiag(mps, jobs, st):
 
 
 def finish_gram(mps, jobs):
-    """Exact Gram factorization of several updates.  Gram matrices of equal
-    order are factored together: orders <= 64 in the batched Jacobi kernel,
-    larger ones in one strided-batched rocSOLVER call per order, the Gram
-    GEMMs writing straight into the batch the eigensolver overwrites.  A
-    matrix the batched call leaves unconverged is rebuilt and retried alone
-    (``eigh_desc``: jittered retry), then falls back to the plain SVD."""
-    from .eig import JACOBI_MAX, heevd_batched
+    """Exact Gram factorization of several updates.  Complex64 updates of one
+    order above 64 with m <= n are factored together (``_gram_group_tridiag``:
+    one launch per step for the whole group); every other update gets its own
+    Gram matrix and ``TopEig`` (the Jacobi kernel up to order 64, else the
+    tridiagonal path with the kept vectors only).  If the eigensolver fails
+    (vectors failing the orthogonality re-check and rocSOLVER's jittered
+    retries after them), the update takes the plain method's SVD cascade,
+    precision promotion included."""
+    from .eig import JACOBI_MAX
     st = _factor_stats(mps)
     orders = {}
     for idx, j in enumerate(jobs):
         orders.setdefault(min(j.X.shape), []).append(idx)
-    grams = [None] * len(jobs)       # (G or None, left, X64)
-    eig = [None] * len(jobs)
     done = {}
     for n, idxs in orders.items():
         grouped = [i for i in idxs if jobs[i].X.shape[0] <= jobs[i].X.shape[1]
@@ -467,29 +460,6 @@ def finish_gram(mps, jobs):
             for i, r in zip(grouped, res):
                 if r is not None:
                     done[i] = r
-            idxs = [i for i in idxs if i not in done]
-            if not idxs:
-                continue
-        if n <= JACOBI_MAX or len(idxs) == 1:
-            for idx in idxs:
-                grams[idx] = _gram_matrix(jobs[idx].X)
-                try:
-                    eig[idx] = eigh_desc(grams[idx][0])
-                except RuntimeError:
-                    pass
-            continue
-        Gb = torch.empty((len(idxs), n, n), dtype=torch.complex128, device=jobs[idxs[0]].X.device)
-        for b, idx in enumerate(idxs):
-            _, left, X64 = _gram_matrix(jobs[idx].X, out=Gb[b])
-            grams[idx] = (None, left, X64)       # Gb[b] is overwritten below
-        # streamed bytes of the one-stage tridiagonalisation: the trailing
-        # n-j x n-j block of every matrix is read once per column j
-        with RL.timed("eig_zheevd_batched", len(idxs) * (16.0 / 3.0 + 4.0 / 3.0 + 8.0) * n ** 3,
-                      "fp64_mfma"):
-            lam, V, ok = heevd_batched(Gb, inplace=True, return_ok=True)
-        for b, idx in enumerate(idxs):
-            if ok[b]:
-                eig[idx] = (lam[b], V[b])
     outs = []
     for idx, j in enumerate(jobs):
         if idx in done:
@@ -497,24 +467,20 @@ def finish_gram(mps, jobs):
             outs.append(done[idx])
             _harvest(mps, j.feats, done[idx][2])
             continue
-        G, left, X64 = grams[idx]
-        if eig[idx] is None:
-            try:
-                if G is None:
-                    G = _gram_matrix(j.X)[0]
-                eig[idx] = eigh_desc(G)
-            except RuntimeError:
-                # the FP64 eigensolver failed twice: the plain method's SVD cascade
-                res = plain_factor(mps, j.X, j.cap, j.budget_ok, j.local, j.cl, j.cr)
-                st.bump("plain")
-                outs.append(res)
-                _harvest(mps, j.feats, res[2])
-                continue
-        lam, V = eig[idx]
-        S = torch.sqrt(torch.clamp(lam, min=0.0))
-        U, Vr = (V, None) if left else (None, V)
-        A_new, B_new, k, e, s, c, dt = _finish_from_gram(mps, j.X, U, S, Vr, X64, j.local, j.cap,
-                                                        j.budget_ok, j.cl, j.cr, mps.eps_bond)
+        try:
+            G, left, X64 = _gram_matrix(j.X)
+            te = TopEig(G)
+            S = torch.sqrt(torch.clamp(te.values, min=0.0))
+            A_new, B_new, k, e, s, c, dt = _finish_from_gram(
+                mps, j.X, te.vectors, left, S, X64, j.local, j.cap, j.budget_ok, j.cl, j.cr,
+                mps.eps_bond)
+        except RuntimeError:
+            # the FP64 eigensolver failed: the plain method's SVD cascade
+            res = plain_factor(mps, j.X, j.cap, j.budget_ok, j.local, j.cl, j.cr)
+            st.bump("plain")
+            outs.append(res)
+            _harvest(mps, j.feats, res[2])
+            continue
         st.bump("gram")
         outs.append((A_new, B_new, k, e, s, c, "gram_fp64", dt))
         _harvest(mps, j.feats, k)

@@ -90,9 +90,20 @@ def theta(A: torch.Tensor, B: torch.Tensor, gate64: np.ndarray, prec: str = "nat
     assert out.is_contiguous() and tuple(out.shape) == (2 * cl, 2 * cr) and out.dtype == A.dtype
     g = np.ascontiguousarray(gate64, dtype=np.float64)
     assert g.size == 32
+    if prec == "bf16x3" and A.dtype == torch.complex64 and THETA_PRESPLIT:
+        # operands split once into BF16 planes (workspace from the caching
+        # allocator, released in stream order after the launch)
+        nbytes = int(lib().aq_theta_planes_bytes(cl, cm, cr))
+        ws = torch.empty(nbytes, dtype=torch.uint8, device=A.device)
+        check(lib().aq_theta_planes(_ptr(A), _ptr(B), _dptr(g), _ptr(out), cl, cm, cr, _ptr(ws),
+                                    nbytes, _stream(A)), "aq_theta_planes")
+        return out
     check(lib().aq_theta(_code(A.dtype), PREC[prec], _ptr(A), _ptr(B), _dptr(g), _ptr(out),
                          cl, cm, cr, _stream(A)), "aq_theta")
     return out
+
+
+THETA_PRESPLIT = True   # BF16x3 theta on operands split once (aq_theta_planes)
 
 
 # ---------------------------------------------------------------- rank rule

@@ -5,8 +5,11 @@ For X [m, n] (complex64) and a width l = k_guess + p, p = max(16, k_guess):
   q x { Z = X^H Q ; Y = X Z' }     optional power steps (q = POWER_STEPS),
                                    re-orthonormalised
   Q = orth(Y)                      shifted Cholesky-QR, twice, in FP64
+                                   (the library's aq_cholqr)
   B = Q^H X        [l, n]          FP64 MFMA GEMM
-  B B^H = U_B diag(lam) U_B^H      FP64 Hermitian core
+  B B^H = U_B diag(lam) U_B^H      FP64 Hermitian core (Jacobi kernel up to
+                                   order 64, else the tridiagonal path; only
+                                   the kept k vectors are formed)
   tail = ||X||_F^2 - sum(lam)      FP64: weight outside range(Q)
 The captured eigenvalues bound the true squared singular values:
 sigma_i^2 >= lam_i (Cauchy interlacing for the projection Q^H X) and
@@ -23,6 +26,7 @@ Otherwise the caller falls back to the exact Gram factorization.
 """
 from __future__ import annotations
 
+import ctypes
 import math
 
 import numpy as np
@@ -30,7 +34,7 @@ import torch
 
 from . import kernels as K
 from . import roofline as RL
-from .eig import eigh_desc
+from .eig import TopEig
 from .truncation import choose_rank_from_sigma
 
 FID_TOL = 1e-7         # admissible uncaptured weight, of ||X||_F^2
@@ -110,26 +114,34 @@ def orth(Y: torch.Tensor, infos: list) -> torch.Tensor:
     """Orthonormal basis of the columns of Y [m, l] (complex128) by shifted
     Cholesky-QR followed by Cholesky-QR2 (shiftedCholeskyQR3: Fukaya,
     Kannan, Nakatsukasa, Yamamoto, Yanagisawa, SIAM J. Sci. Comput. 42 (2020)),
-    with their shift 11 (m l + l (l + 1)) u ||Y||^2 (||Y||_F^2 as the bound).
-    The first pass keeps the span of Y and makes the next two well posed.  No
-    host round trip: the factorizations' info codes are appended to `infos`
-    and checked once by the caller.  (The library's Householder QR,
-    ``kernels.qr``, would need no shift, but its 32-column panel
-    factorisation is a chain of column steps: on the sketches here, m = 2048
-    and l ~ 100-250, three of them per update halved the low-entanglement
-    rate, 192 -> 86 updates/s; the Cholesky factor of the small l x l Gram
-    matrix costs far less.)"""
+    with their shift 11 (m l + l (l + 1)) u ||Y||^2 (||Y||_F^2 as the bound),
+    each pass one ``aq_cholqr`` call on the library's kernels (Gram matrix and
+    triangular solves on MFMA GEMMs, 64 x 64 diagonal blocks factored and
+    inverted in LDS).  The first pass keeps the span of Y and makes the next
+    two well posed.  No host round trip: the passes' pivot codes (int32 [3]
+    on the device) are appended to `infos` and checked once by the caller.
+    (The library's Householder QR, ``kernels.qr``, would need no shift, but
+    its 32-column panel factorisation is a chain of column steps: on the
+    sketches here, m = 2048 and l ~ 100-250, it halved the low-entanglement
+    rate; the small l x l Cholesky factors cost far less.)"""
+    from ._ffi import check, lib
     m, l = Y.shape
     u = 2.0 ** -53
+    Y = Y.contiguous()
+    L = lib()
+    nws = int(L.aq_cholqr_workspace_elems(m, l))
+    ws = torch.empty(nws, dtype=torch.complex128, device=Y.device)
+    info = torch.zeros(3, dtype=torch.int32, device=Y.device)
+    bufs = (torch.empty_like(Y), torch.empty_like(Y))
+    src = Y
     for it in range(3):
-        W = K.cgemm(Y.T, Y, conj_a=True)
-        if it == 0:
-            shift = 11.0 * (m * l + l * (l + 1)) * u * torch.diagonal(W).real.sum()
-            W = W + shift * torch.eye(l, dtype=W.dtype, device=W.device)
-        L, info = torch.linalg.cholesky_ex(W)
-        infos.append(info)
-        Y = torch.linalg.solve_triangular(L.conj().T, Y, upper=True, left=False)
-    return Y
+        dst = bufs[it % 2]
+        scale = 11.0 * (m * l + l * (l + 1)) * u if it == 0 else 0.0
+        check(L.aq_cholqr(m, l, K._ptr(src), l, K._ptr(dst), l, scale, K._ptr(ws), nws,
+                          ctypes.c_void_p(info.data_ptr() + 4 * it), K._stream(Y)), "aq_cholqr")
+        src = dst
+    infos.append(info)
+    return src
 
 
 _TLS = __import__("threading").local()
@@ -181,8 +193,8 @@ def try_rsvd(mps, i, X, cap, budget_ok, local, cl, cr, st, k_guess):
     with RL.timed("rsvd_project_fp64", 8.0 * l * m * n, "fp64_mfma"):
         Bc = K.cgemm(Q.T, X64, conj_a=True)                       # [l, n]
     G = K.cgemm(Bc, Bc.T, conj_b=True)                           # [l, l]
-    lam, UB = eigh_desc(G)
-    lam = torch.clamp(lam, min=0.0)
+    core = TopEig(G)            # eigenvalues now, the kept vectors after the rank rule
+    lam = torch.clamp(core.values, min=0.0)
     # one device-to-host transfer: captured spectrum, ||X||^2, factorization codes
     host = torch.cat([lam, torch.vdot(X64.reshape(-1), X64.reshape(-1)).real.view(1),
                       torch.stack(infos).to(torch.float64).abs().sum().view(1)]).cpu().numpy()
@@ -218,8 +230,9 @@ def try_rsvd(mps, i, X, cap, budget_ok, local, cl, cr, st, k_guess):
         return None, "condition"  # promotion semantics: let the exact path decide in FP64
     Sk = S[:k]
     inv = torch.where(Sk > 0, 1.0 / Sk, torch.zeros_like(Sk))
-    Uk = K.cgemm(Q, UB[:, :k].contiguous())                       # [m, k]
-    Vh = K.cgemm(UB[:, :k].T, Bc, conj_a=True) * inv[:, None]     # [k, n]
+    UBk = core.vectors(k).contiguous()                            # [l, k]
+    Uk = K.cgemm(Q, UBk)                                          # [m, k]
+    Vh = K.cgemm(UBk.T, Bc, conj_a=True) * inv[:, None]           # [k, n]
     A_new, B_new = K.rebuild(Uk.to(local), Sk.to(torch.float32), Vh.to(local).contiguous(),
                              k, cl, cr)
     st.bump("rsvd")

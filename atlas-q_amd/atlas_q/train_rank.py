@@ -9,10 +9,10 @@ features of the theta matrix are recorded, after it the rank the reference's
 rule kept.  Circuit families: Haar brickwork, weakly entangling brickwork
 (exp(-i a H), H Hermitian Gaussian, a in [0.03, 0.6]) and the RY/RZ + CZ
 hardware-efficient ansatz, over chain lengths 8-40, caps 8-256 and cutoffs
-1e-2 .. 1e-8; and steady-state slices at bond 32-256 with flat or decaying
-Schmidt profiles (``atlas_q.workloads``) at bond 32-1024 under Haar or weak
-gates (bond >= 512 harvested with the exact Gram method, whose ranks are the
-plain method's up to float32 rounding).  Split by circuit (90/10), as the
+1e-2 .. 1e-8; and steady-state slices with flat or decaying Schmidt
+profiles (``atlas_q.workloads``) at bond 32-2048 under Haar or weak gates
+(bond >= 512 harvested with the exact Gram method, whose ranks are the plain
+method's up to float32 rounding), so kept ranks up to 2048 are in the data.  Split by circuit (90/10), as the
 reference's dataset builder
 splits by file (scripts/ai_tools/build_ft_dataset.py:103-108).
 """
@@ -45,7 +45,7 @@ def slice_job(rng):
     """Steady-state slices (bonds chi at both ends) with flat or decaying
     Schmidt profiles, Haar or weakly entangling gates."""
     from .workloads import gate_schedule, slice_tensors
-    chi = int(2 ** rng.integers(5, 11))
+    chi = int(2 ** rng.integers(5, 12))
     kind = "flat" if rng.random() < 0.3 else "decay"
     tau = float(rng.uniform(3.0, 40.0))
     angle = None if rng.random() < 0.3 else float(rng.uniform(0.02, 0.6))

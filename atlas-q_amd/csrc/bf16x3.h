@@ -166,3 +166,132 @@ __device__ __forceinline__ void cgemm_mainloop_bf16x3(const AOp& aop, const BOp&
 }
 
 }  // namespace aq
+
+// ---------------------------------------------------------------- pre-split planes
+// The operands split once, before the GEMM, into four BF16 planes (re_hi,
+// re_lo, im_hi, im_lo), k-contiguous and zero-padded to whole tiles, so the
+// mainloop stages each k step with 16-byte copies (no per-tile conversion,
+// no bounds checks):
+//   A planes  P[plane][m][k],     m < Mp (multiple of kBM), k < Kp (of kBKb)
+//   B planes  P[plane][h][j][k],  h < 2 halves, j < Jp (multiple of kBJ)
+// For a linear (unpaired) B the two halves are the left and right kBJ
+// columns of every kBN-column tile: column n = (n / kBN) kBN + h kBJ + j'.
+namespace aq {
+
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+typedef const __attribute__((address_space(1))) u32x4* gptr_u32x4;
+
+struct PlanesA {
+  const uint16_t* p;
+  int64_t Kp, plane;     // row stride (elements), plane stride
+};
+struct PlanesB {
+  const uint16_t* p;
+  int64_t Kp, plane, half;  // column stride, plane stride, half stride
+};
+
+__device__ __forceinline__ void split8(const float2 (&x)[8], uint4 (&o)[4]) {
+  uint16_t h[4][8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    split_bf16(x[e].x, h[0][e], h[1][e]);
+    split_bf16(x[e].y, h[2][e], h[3][e]);
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+    o[q] = make_uint4(h[q][0] | ((uint32_t)h[q][1] << 16), h[q][2] | ((uint32_t)h[q][3] << 16),
+                      h[q][4] | ((uint32_t)h[q][5] << 16), h[q][6] | ((uint32_t)h[q][7] << 16));
+}
+
+// The mainloop on pre-split planes: tile rows [m0, m0 + kBM), columns j0 ..
+// j0 + kBJ of both halves; K padded (Kp a multiple of kBKb, zeros beyond K).
+__device__ __forceinline__ void cgemm_mainloop_planes(const PlanesA& pa, const PlanesB& pb,
+                                                      int64_t Kp, int64_t m0, int64_t j0,
+                                                      SmemBf16& sm, Frag<float>& fr) {
+  constexpr int BK = kBKb;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      fr.re[0][0][h][r] = 0;
+      fr.im[0][0][h][r] = 0;
+    }
+  // 16-byte chunks: A 4 planes x 64 rows x 4 = 1024 (4 per thread),
+  // B 4 planes x 2 halves x 64 columns x 4 = 2048 (8 per thread)
+  gptr_u32x4 ga[4];
+  gptr_u32x4 gb[8];
+  uint16_t* la[4];
+  uint16_t* lb[8];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int c = tid + 256 * i;
+    const int q = c >> 8, row = (c >> 2) & 63, kq = c & 3;
+    ga[i] = (gptr_u32x4)(pa.p + q * pa.plane +
+                                                            (m0 + row) * pa.Kp + 8 * kq);
+    la[i] = &sm.a[q][row][8 * kq];
+  }
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const int c = tid + 256 * i;
+    const int q = c >> 9, h = (c >> 8) & 1, col = (c >> 2) & 63, kq = c & 3;
+    gb[i] = (gptr_u32x4)(pb.p + q * pb.plane + h * pb.half +
+                                                            (j0 + col) * pb.Kp + 8 * kq);
+    lb[i] = &sm.b[q][h * kBJ + col][8 * kq];
+  }
+  u32x4 ra[4], rb[8];
+  auto fetch = [&](int64_t k0) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) ra[i] = ga[i][k0 / 8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) rb[i] = gb[i][k0 / 8];
+  };
+  fetch(0);
+  for (int64_t k0 = 0; k0 < Kp; k0 += BK) {
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 4; ++i) *reinterpret_cast<u32x4*>(la[i]) = ra[i];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) *reinterpret_cast<u32x4*>(lb[i]) = rb[i];
+    __syncthreads();
+    if (k0 + BK < Kp) fetch(k0 + BK);
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 16) {
+      const int kof = kk + 8 * (lane >> 5);
+      const int row = 32 * wm + (lane & 31);
+      const bf16x8 arh = load8(&sm.a[0][row][kof]);
+      const bf16x8 arl = load8(&sm.a[1][row][kof]);
+      const bf16x8 aih = load8(&sm.a[2][row][kof]);
+      const bf16x8 ail = load8(&sm.a[3][row][kof]);
+      const bf16x8 naih = neg8(aih), nail = neg8(ail);
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int col = h * kBJ + 32 * wn + (lane & 31);
+        const bf16x8 brh = load8(&sm.b[0][col][kof]);
+        const bf16x8 brl = load8(&sm.b[1][col][kof]);
+        const bf16x8 bih = load8(&sm.b[2][col][kof]);
+        const bf16x8 bil = load8(&sm.b[3][col][kof]);
+        f32x16 cr = fr.re[0][0][h], ci = fr.im[0][0][h];
+        cr = mma_bf16(arh, brl, cr);
+        cr = mma_bf16(arl, brh, cr);
+        cr = mma_bf16(naih, bil, cr);
+        cr = mma_bf16(nail, bih, cr);
+        cr = mma_bf16(arh, brh, cr);
+        cr = mma_bf16(naih, bih, cr);
+        ci = mma_bf16(arh, bil, ci);
+        ci = mma_bf16(arl, bih, ci);
+        ci = mma_bf16(aih, brl, ci);
+        ci = mma_bf16(ail, brh, ci);
+        ci = mma_bf16(arh, bih, ci);
+        ci = mma_bf16(aih, brh, ci);
+        fr.re[0][0][h] = cr;
+        fr.im[0][0][h] = ci;
+      }
+    }
+  }
+}
+
+}  // namespace aq

@@ -9,6 +9,8 @@
 // gate is applied per lane in registers and X is written once, coalesced
 // along j.  Replaces fused_two_qubit_gate_pytorch
 // (src/triton_kernels/mps_complex.py:343-368).
+#include <algorithm>
+
 #include "gemm_core.h"
 #include "bf16x3.h"
 
@@ -93,6 +95,84 @@ __global__ __launch_bounds__(kThreads) void theta_bf16x3_kernel(
   ColMap cm{1, chiR};
   Frag<float> fr;
   cgemm_mainloop_bf16x3<true, true>(aop, bop, cm, chiM, m0, j0, sm, fr);
+  theta_epilogue<float>(fr, g, X, chiL, chiR, m0, j0);
+}
+
+// A(m, k) = A[m lda + k] (k contiguous) -> planes; one thread per 8 k values.
+__global__ __launch_bounds__(256) void split_rows_kernel(const float2* __restrict__ A, int64_t M,
+                                                         int64_t K, int64_t lda, int conj,
+                                                         uint16_t* __restrict__ P, int64_t Mp,
+                                                         int64_t Kp) {
+  const int64_t kq = Kp / 8;
+  const int64_t total = Mp * kq;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t m = idx / kq, k0 = (idx % kq) * 8;
+    float2 x[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const bool ok = m < M && k0 + e < K;
+      x[e] = ok ? A[m * lda + k0 + e] : make_float2(0.f, 0.f);
+      if (conj) x[e].y = -x[e].y;
+    }
+    uint4 o[4];
+    split8(x, o);
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      *reinterpret_cast<uint4*>(P + q * Mp * Kp + m * Kp + k0) = o[q];
+  }
+}
+
+// B(k, n) = B[k ldb + n] (n contiguous) -> transposed planes, through a
+// 32 (k) x 64 (j) LDS tile per work-group: reads coalesced along n, writes
+// along k.  grid = (Kp / 32, Jp / 64, 2 halves); paired: column h half_n + j,
+// else (j / kBJ) kBN + h kBJ + j % kBJ.
+__global__ __launch_bounds__(256) void split_cols_kernel(const float2* __restrict__ B, int64_t K,
+                                                         int64_t N, int64_t ldb, int paired,
+                                                         int64_t half_n, uint16_t* __restrict__ P,
+                                                         int64_t Jp, int64_t Kp) {
+  __shared__ float2 t[32][64 + 1];
+  const int64_t k0 = (int64_t)blockIdx.x * 32, jb = (int64_t)blockIdx.y * 64;
+  const int h = blockIdx.z;
+  const int tid = threadIdx.x;
+  for (int e = tid; e < 32 * 64; e += 256) {
+    const int kk = e / 64, jj = e % 64;
+    const int64_t j = jb + jj, k = k0 + kk;
+    int64_t n;
+    bool ok;
+    if (paired) {
+      n = h * half_n + j;
+      ok = j < half_n;
+    } else {
+      n = (j / kBJ) * kBN + h * kBJ + j % kBJ;
+      ok = n < N;
+    }
+    t[kk][jj] = (ok && k < K) ? B[k * ldb + n] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  // 64 columns x 4 groups of 8 k per thread
+  const int jj = tid / 4, kq = tid % 4;
+  float2 x[8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) x[e] = t[8 * kq + e][jj];
+  uint4 o[4];
+  split8(x, o);
+  const int64_t off = (int64_t)h * Jp * Kp + (jb + jj) * Kp + k0 + 8 * kq;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) *reinterpret_cast<uint4*>(P + q * 2 * Jp * Kp + off) = o[q];
+}
+
+// BF16x3 theta on operands split beforehand (split_rows_kernel /
+// split_cols_kernel): the mainloop only copies 16-byte chunks into LDS.
+__global__ __launch_bounds__(kThreads, 2) void theta_planes_kernel(PlanesA pa, PlanesB pb,
+                                                                 int64_t Kp, Gate4<float> g,
+                                                                 float2* __restrict__ X,
+                                                                 int64_t chiL, int64_t chiR) {
+  __shared__ SmemBf16 sm;
+  const int64_t m0 = (int64_t)blockIdx.x * kBM;
+  const int64_t j0 = (int64_t)blockIdx.y * kBJ;
+  Frag<float> fr;
+  cgemm_mainloop_planes(pa, pb, Kp, m0, j0, sm, fr);
   theta_epilogue<float>(fr, g, X, chiL, chiR, m0, j0);
 }
 
@@ -311,6 +391,65 @@ extern "C" int aq_theta(int dtype, int prec, const void* A, const void* B,
   } else {
     AQ_REQUIRE(false, AQ_ERR_ARG, "aq_theta: unknown dtype %d", dtype);
   }
+  AQ_LAUNCH_CHECK();
+  return AQ_OK;
+}
+
+namespace aq {
+struct ThetaPlanes {
+  int64_t Mp, Jp, Kp, a_elems, b_elems;
+};
+static ThetaPlanes theta_planes(int64_t chiL, int64_t chiM, int64_t chiR) {
+  ThetaPlanes t;
+  t.Mp = cdiv(2 * chiL, kBM) * kBM;
+  t.Jp = cdiv(chiR, kBJ) * kBJ;
+  t.Kp = cdiv(chiM, kBKb) * kBKb;
+  t.a_elems = 4 * t.Mp * t.Kp;
+  t.b_elems = 4 * 2 * t.Jp * t.Kp;
+  return t;
+}
+}  // namespace aq
+
+extern "C" int64_t aq_theta_planes_bytes(int64_t chiL, int64_t chiM, int64_t chiR) {
+  if (chiL <= 0 || chiM <= 0 || chiR <= 0) return 0;
+  const ThetaPlanes t = theta_planes(chiL, chiM, chiR);
+  return 2 * (t.a_elems + t.b_elems);
+}
+
+extern "C" int aq_theta_planes(const void* A, const void* B, const double* gate_host, void* X,
+                               int64_t chiL, int64_t chiM, int64_t chiR, void* ws,
+                               int64_t ws_bytes, hipStream_t stream) {
+  AQ_REQUIRE(A && B && X && gate_host && ws, AQ_ERR_ARG, "aq_theta_planes: null pointer");
+  AQ_REQUIRE(chiL > 0 && chiM > 0 && chiR > 0, AQ_ERR_ARG,
+             "aq_theta_planes: bond sizes must be positive (got %lld, %lld, %lld)",
+             (long long)chiL, (long long)chiM, (long long)chiR);
+  AQ_REQUIRE(chiL <= (1 << 20) && chiR <= (1 << 20) && chiM <= (1 << 20), AQ_ERR_ARG,
+             "aq_theta_planes: bond size too large");
+  AQ_REQUIRE(((uintptr_t)ws & 15) == 0, AQ_ERR_ARG, "aq_theta_planes: workspace not 16-byte aligned");
+  const ThetaPlanes t = theta_planes(chiL, chiM, chiR);
+  AQ_REQUIRE(ws_bytes >= 2 * (t.a_elems + t.b_elems), AQ_ERR_ARG,
+             "aq_theta_planes: workspace of %lld bytes, %lld needed", (long long)ws_bytes,
+             (long long)(2 * (t.a_elems + t.b_elems)));
+  uint16_t* Pa = (uint16_t*)ws;
+  uint16_t* Pb = Pa + t.a_elems;
+  {
+    const int64_t total = t.Mp * t.Kp / 8;
+    const unsigned grid = (unsigned)std::min<int64_t>(cdiv(total, 256), 65536);
+    split_rows_kernel<<<grid, 256, 0, stream>>>((const float2*)A, 2 * chiL, chiM, chiM, 0, Pa,
+                                                t.Mp, t.Kp);
+    AQ_LAUNCH_CHECK();
+  }
+  {
+    dim3 grid((unsigned)(t.Kp / 32), (unsigned)(t.Jp / 64), 2);
+    split_cols_kernel<<<grid, 256, 0, stream>>>((const float2*)B, chiM, 2 * chiR, 2 * chiR, 1,
+                                                chiR, Pb, t.Jp, t.Kp);
+    AQ_LAUNCH_CHECK();
+  }
+  PlanesA pa{Pa, t.Kp, t.Mp * t.Kp};
+  PlanesB pb{Pb, t.Kp, 2 * t.Jp * t.Kp, t.Jp * t.Kp};
+  dim3 grid((unsigned)(t.Mp / kBM), (unsigned)(t.Jp / kBJ));
+  theta_planes_kernel<<<grid, kThreads, 0, stream>>>(pa, pb, t.Kp, make_gate<float>(gate_host),
+                                                      (float2*)X, chiL, chiR);
   AQ_LAUNCH_CHECK();
   return AQ_OK;
 }

@@ -146,6 +146,34 @@ __global__ __launch_bounds__(kTriThreads) void tri_bisect_kernel(
 }
 
 // ---------------------------------------------------------------- inverse iteration
+// The unreduced block [s, t] (split at zero off-diagonals) of the j-th
+// largest eigenvalue: copies of a repeated eigenvalue (within delta) are
+// taken from different blocks, in block order.
+__device__ __forceinline__ void find_block(int n, const double* d, const double* e2,
+                                           const double* e, const double* Wb, int j,
+                                           double delta, double pivmin, int& s, int& t) {
+  const double lam = Wb[j];
+  int j0 = j;
+  while (j0 > 0 && Wb[j0 - 1] - lam <= delta) --j0;
+  const int copy = j - j0;
+  int acc = 0, bs = 0;
+  s = 0;
+  t = n - 1;
+  for (int i = 0; i < n; ++i) {
+    if (i == n - 1 || e[i] == 0.0) {  // block [bs, i]
+      const int c = sturm_count(d, e2, bs, i, lam + delta, pivmin) -
+                    sturm_count(d, e2, bs, i, lam - delta, pivmin);
+      if (acc + c > copy) {
+        s = bs;
+        t = i;
+        return;
+      }
+      acc += c;
+      bs = i + 1;
+    }
+  }
+}
+
 // Zt[b, j, :] = unit eigenvector (real, stored as complex) of W[b, j] for
 // j < k[b]; rows k[b] .. kmax-1 are zeroed.  scratch: 3 * n * batch * kmax
 // doubles (L, U, D+ per thread, thread-minor so that steps coalesce).
@@ -176,26 +204,7 @@ __global__ __launch_bounds__(kTriThreads) void tri_invit_kernel(
   const double delta = 8.0 * DBL_EPSILON * tb.tnorm + 8.0 * tb.pivmin;
   // block of the tridiagonal this vector lives in
   int s = 0, t = n - 1;
-  if (nsplit > 0) {
-    int j0 = j;
-    while (j0 > 0 && Wb[j0 - 1] - lam <= delta) --j0;
-    const int copy = j - j0;
-    int acc = 0, bs = 0;
-    bool found = false;
-    for (int i = 0; i < n && !found; ++i) {
-      if (i == n - 1 || e[i] == 0.0) {  // block [bs, i]
-        const int c = sturm_count(d, e2, bs, i, lam + delta, tb.pivmin) -
-                      sturm_count(d, e2, bs, i, lam - delta, tb.pivmin);
-        if (acc + c > copy) {
-          s = bs;
-          t = i;
-          found = true;
-        }
-        acc += c;
-        bs = i + 1;
-      }
-    }
-  }
+  if (nsplit > 0) find_block(n, d, e2, e, Wb, j, delta, tb.pivmin, s, t);
   const int64_t T = batch * (int64_t)kmax;
   const int64_t tid = b * kmax + j;
   double* Ls = scratch;
@@ -252,6 +261,198 @@ __global__ __launch_bounds__(kTriThreads) void tri_invit_kernel(
     z[i + 1] = make_double2(zi * inv, 0.0);
   }
 #undef AQ_SC
+}
+
+// ---------------------------------------------------------------- clusters
+// Vectors of clustered eigenvalues.  One twisted-factorisation step gives an
+// eigenvector accurate to about eps ||T|| / gap (measured: 1e-10 .. 3e-10
+// non-orthogonality at gaps of 1e-6 .. 2e-6 ||T||), and copies of one value
+// in one block the same vector.  Kept eigenvalues of one unreduced block
+// that chain within gtol = kClusterRel ||T|| of each other form a cluster;
+// every cluster of two or more is recomputed as a block (LAPACK dstein's
+// inverse iteration, its members at once): one thread per member factors
+// T_block - lambda_r (guarded L D L^T) and runs three solves from a fixed
+// pseudo-random start, normalising its own vector; the work-group then
+// orthonormalises the block by modified Gram-Schmidt, twice (norms as
+// work-group reductions, one thread per later column for the projections).
+// Clusters are taken one after another by the matrix's work-group.  The
+// columns live in scratch as [n][kmax] (one coalesced row per step).
+constexpr double kClusterRel = 1e-5;
+
+__device__ __forceinline__ double start_value(int j, int i) {
+  uint32_t h = (uint32_t)j * 0x9E3779B1u ^ ((uint32_t)i * 0x85EBCA77u + 0x165667B1u);
+  h ^= h >> 15;
+  h *= 0x2C1B3C6Du;
+  h ^= h >> 12;
+  h *= 0x297A2D39u;
+  h ^= h >> 15;
+  return (double)(h & 0xFFFFFu) / 524288.0 - 1.0;
+}
+
+__device__ double wg_sum(double v, double* red) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int q = 0; q < (int)(blockDim.x >> 6); ++q) s += red[q];
+  return s;
+}
+
+// Doubles of scratch per matrix: three [n][kmax] column blocks (L factors,
+// reciprocal pivots, vectors) and ints (block start / end, cluster id, next
+// member, roots, member list per cluster, chain tail per block start).
+__host__ __device__ inline int64_t cluster_scratch_doubles(int64_t n, int64_t kmax) {
+  const int64_t ints = 6 * kmax + n + 8;
+  return 3 * n * kmax + (ints + 1) / 2 + 2;
+}
+
+template <int NMAX>
+__global__ __launch_bounds__(kTriThreads) void tri_cluster_kernel(
+    int n, const double* __restrict__ D, int64_t sD, const double* __restrict__ E, int64_t sE,
+    const double* __restrict__ W, int64_t sW, const int32_t* __restrict__ kvec, int kmax,
+    double2* __restrict__ Z, int64_t sZ, double* __restrict__ scratch, int64_t sScr) {
+  __shared__ double d[NMAX], e2[NMAX], e[NMAX];
+  __shared__ double red[kTriThreads / 64];
+  __shared__ int nsplit, nroots, msize;
+  const int64_t b = blockIdx.x;
+  const TriBounds tb = load_tridiag(D + b * sD, E + b * sE, n, d, e2, e, true);
+  const int kb = kvec ? min((int)kvec[b], kmax) : kmax;
+  if (kb < 2) return;
+  if (threadIdx.x == 0) nsplit = 0;
+  __syncthreads();
+  for (int i = threadIdx.x; i + 1 < n; i += blockDim.x)
+    if (e[i] == 0.0) atomicAdd(&nsplit, 1);
+  __syncthreads();
+  const double* Wb = W + b * sW;
+  double* LF = scratch + b * sScr;                  // [n][kmax]
+  double* DF = LF + (int64_t)n * kmax;
+  double* X = DF + (int64_t)n * kmax;
+  int* bs = (int*)(X + (int64_t)n * kmax);
+  int* bt = bs + kmax;
+  int* cl = bt + kmax;
+  int* nx = cl + kmax;
+  int* roots = nx + kmax;
+  int* mem = roots + kmax;
+  int* last = mem + kmax;                            // [n]
+  const int64_t ld = kmax;
+  const double delta = 8.0 * DBL_EPSILON * tb.tnorm + 8.0 * tb.pivmin;
+  const double gabs = kClusterRel * tb.tnorm;
+  const double pfloor = fmax(DBL_EPSILON * tb.tnorm, tb.pivmin);
+  for (int j = threadIdx.x; j < kb; j += blockDim.x) {
+    int s = 0, t = n - 1;
+    if (nsplit > 0) find_block(n, d, e2, e, Wb, j, delta, tb.pivmin, s, t);
+    bs[j] = s;
+    bt[j] = t;
+    nx[j] = -1;
+  }
+  for (int i = threadIdx.x; i < n; i += blockDim.x) last[i] = -1;
+  __threadfence_block();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    // chains per block: cl[j] = root, nx[] links the members in order
+    int nr = 0;
+    for (int j = 0; j < kb; ++j) {
+      const int p = last[bs[j]];
+      if (p >= 0 && Wb[p] - Wb[j] <= gabs) {
+        cl[j] = cl[p];
+        if (nx[cl[j]] < 0) roots[nr++] = cl[j];   // second member: the root becomes a cluster
+        nx[p] = j;
+      } else {
+        cl[j] = j;
+      }
+      last[bs[j]] = j;
+    }
+    nroots = nr;
+  }
+  __threadfence_block();
+  __syncthreads();
+  double2* Zb = Z + b * sZ;
+  for (int q = 0; q < nroots; ++q) {
+    const int root = roots[q];
+    const int s = bs[root], t = bt[root];
+    if (threadIdx.x == 0) {
+      int m = 0;
+      for (int j = root; j >= 0; j = nx[j]) mem[m++] = j;
+      msize = m;
+    }
+    __threadfence_block();
+    __syncthreads();
+    const int m = msize;
+    // inverse iteration, one thread per member
+    for (int r = threadIdx.x; r < m; r += blockDim.x) {
+      const int j = mem[r];
+      const double lam = Wb[j];
+      double dp = d[s] - lam;
+      if (fabs(dp) < pfloor) dp = dp < 0.0 ? -pfloor : pfloor;
+      DF[(int64_t)s * ld + r] = 1.0 / dp;
+      for (int i = s; i < t; ++i) {
+        const double l = e[i] / dp;
+        LF[(int64_t)i * ld + r] = l;
+        dp = d[i + 1] - lam - l * e[i];
+        if (fabs(dp) < pfloor) dp = dp < 0.0 ? -pfloor : pfloor;
+        DF[(int64_t)(i + 1) * ld + r] = 1.0 / dp;
+      }
+      for (int i = s; i <= t; ++i) X[(int64_t)i * ld + r] = start_value(j, i);
+      for (int it = 0; it < 3; ++it) {
+        double xp = X[(int64_t)s * ld + r];
+        double mx = fabs(xp);
+        for (int i = s + 1; i <= t; ++i) {
+          xp = X[(int64_t)i * ld + r] - LF[(int64_t)(i - 1) * ld + r] * xp;
+          X[(int64_t)i * ld + r] = xp;
+          mx = fmax(mx, fabs(xp));
+        }
+        double sc = mx > 0.0 ? 1.0 / mx : 1.0;
+        mx = 0.0;
+        for (int i = s; i <= t; ++i) {
+          const double v = X[(int64_t)i * ld + r] * sc * DF[(int64_t)i * ld + r];
+          X[(int64_t)i * ld + r] = v;
+          mx = fmax(mx, fabs(v));
+        }
+        sc = mx > 0.0 ? 1.0 / mx : 1.0;
+        xp = X[(int64_t)t * ld + r] * sc;
+        X[(int64_t)t * ld + r] = xp;
+        double ss = xp * xp;
+        for (int i = t - 1; i >= s; --i) {
+          xp = X[(int64_t)i * ld + r] * sc - LF[(int64_t)i * ld + r] * xp;
+          X[(int64_t)i * ld + r] = xp;
+          ss += xp * xp;
+        }
+        const double inv = ss > 0.0 ? 1.0 / sqrt(ss) : 0.0;
+        for (int i = s; i <= t; ++i) X[(int64_t)i * ld + r] *= inv;
+      }
+    }
+    __threadfence_block();
+    __syncthreads();
+    // modified Gram-Schmidt over the block, twice
+    for (int pass = 0; pass < 2; ++pass)
+      for (int r = 0; r < m; ++r) {
+        double ss = 0.0;
+        for (int i = s + threadIdx.x; i <= t; i += blockDim.x) {
+          const double v = X[(int64_t)i * ld + r];
+          ss += v * v;
+        }
+        ss = wg_sum(ss, red);
+        const double inv = ss > 0.0 ? 1.0 / sqrt(ss) : 0.0;
+        for (int i = s + threadIdx.x; i <= t; i += blockDim.x) X[(int64_t)i * ld + r] *= inv;
+        __threadfence_block();
+        __syncthreads();
+        for (int c = r + 1 + threadIdx.x; c < m; c += blockDim.x) {
+          double h = 0.0;
+          for (int i = s; i <= t; ++i) h += X[(int64_t)i * ld + r] * X[(int64_t)i * ld + c];
+          for (int i = s; i <= t; ++i) X[(int64_t)i * ld + c] -= h * X[(int64_t)i * ld + r];
+        }
+        __threadfence_block();
+        __syncthreads();
+      }
+    for (int64_t e0 = threadIdx.x; e0 < (int64_t)m * n; e0 += blockDim.x) {
+      const int r = (int)(e0 / n), i = (int)(e0 % n);
+      Zb[(int64_t)mem[r] * n + i] = make_double2(i >= s && i <= t ? X[(int64_t)i * ld + r] : 0.0,
+                                                 0.0);
+    }
+    __threadfence_block();
+    __syncthreads();
+  }
 }
 
 // ---------------------------------------------------------------- back-transform blocks
@@ -403,7 +604,8 @@ extern "C" int aq_tridiag_eigvals(int64_t n, const double* D, int64_t sD, const 
 }
 
 extern "C" int64_t aq_tridiag_eigvecs_scratch_doubles(int64_t n, int64_t kmax, int64_t batch) {
-  return 3 * n * kmax * batch;
+  const int64_t kc = std::min<int64_t>(kmax, n);
+  return std::max<int64_t>(3 * n * kmax, cluster_scratch_doubles(n, kc)) * batch;
 }
 
 extern "C" int aq_tridiag_eigvecs(int64_t n, const double* D, int64_t sD, const double* E,
@@ -419,7 +621,7 @@ extern "C" int aq_tridiag_eigvecs(int64_t n, const double* D, int64_t sD, const 
              (long long)kmax);
   AQ_REQUIRE(batch >= 1 && batch <= 65535 && sZ >= kmax * n, AQ_ERR_ARG,
              "aq_tridiag_eigvecs: bad batch or stride");
-  AQ_REQUIRE(scratch_doubles >= 3 * n * kmax * batch, AQ_ERR_ARG,
+  AQ_REQUIRE(scratch_doubles >= aq_tridiag_eigvecs_scratch_doubles(n, kmax, batch), AQ_ERR_ARG,
              "aq_tridiag_eigvecs: scratch too small");
   dim3 grid((unsigned)cdiv(kmax, kTriThreads), (unsigned)batch);
   if (n <= 2048)
@@ -429,6 +631,18 @@ extern "C" int aq_tridiag_eigvecs(int64_t n, const double* D, int64_t sD, const 
     tri_invit_kernel<kTriMaxN><<<grid, kTriThreads, 0, stream>>>(
         (int)n, D, sD, E, sE, W, sW, kvec, (int)kmax, (double2*)Z, sZ, scratch, batch);
   AQ_LAUNCH_CHECK();
+  // clustered eigenvalues: the scratch of the step above is free again
+  const int64_t kc = std::min<int64_t>(kmax, n);
+  const int64_t sScr = cluster_scratch_doubles(n, kc);
+  if (kc >= 2) {
+    if (n <= 2048)
+      tri_cluster_kernel<2048><<<(unsigned)batch, kTriThreads, 0, stream>>>(
+          (int)n, D, sD, E, sE, W, sW, kvec, (int)kc, (double2*)Z, sZ, scratch, sScr);
+    else
+      tri_cluster_kernel<kTriMaxN><<<(unsigned)batch, kTriThreads, 0, stream>>>(
+          (int)n, D, sD, E, sE, W, sW, kvec, (int)kc, (double2*)Z, sZ, scratch, sScr);
+    AQ_LAUNCH_CHECK();
+  }
   return AQ_OK;
 }
 

@@ -124,6 +124,8 @@ def parse():
                     help="skip the copy / rocBLAS GEMM reference rates")
     ap.add_argument("--wide-only", action="store_true")
     ap.add_argument("--batched-only", action="store_true")
+    ap.add_argument("--batched-eig", default="tridiag", choices=("tridiag", "heevd"),
+                    help="eigensolver of the batched form's buckets above order 64")
     ap.add_argument("--mark", action="store_true",
                     help="launch a one-element zero-fill kernel right before and after the "
                          "timed fast-path region (dispatch markers for trace filtering)")
@@ -560,16 +562,20 @@ def batched_bench(torch, a, dev):
     circs = [hea(n, a.batched_layers, a.seed + 100 + j) for j, (n, _) in enumerate(specs)]
     n2q = sum(sum(1 for op in c if op[0] == "2q") for c in circs)
     n1q = sum(sum(1 for op in c if op[0] == "1q") for c in circs)
-    BatchedMPS(specs, eps_bond=1e-8, device=dev).run_hea(circs)      # warm-up / autotune
+    t0 = time.perf_counter()
+    BatchedMPS(specs, eps_bond=1e-8, device=dev,
+               eig_solver=a.batched_eig).run_hea(circs)      # warm-up / autotune
     torch.cuda.synchronize()
+    log(f"batched: warm-up run {time.perf_counter() - t0:.2f}s")
     runs = []
     for _ in range(3):                 # the median of three runs from |0...0>
-        bm = BatchedMPS(specs, eps_bond=1e-8, device=dev)
+        bm = BatchedMPS(specs, eps_bond=1e-8, device=dev, eig_solver=a.batched_eig)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         bm.run_hea(circs)
         torch.cuda.synchronize()
         runs.append(time.perf_counter() - t0)
+        log(f"batched: run {runs[-1]:.3f}s")
     t_circ = sorted(runs)[1]
     S = a.batched_shots
     bm.prepare_sampler(S)
@@ -593,6 +599,7 @@ def batched_bench(torch, a, dev):
         g.replay()
     torch.cuda.synchronize()
     t_samp = (time.perf_counter() - t0) / reps
+    log(f"batched: sampling {t_samp * 1e3:.1f} ms per sweep")
     ref = None
     fids = []
     if a.batched_ref > 0:
@@ -616,6 +623,7 @@ def batched_bench(torch, a, dev):
             fids.append({"qubits": n, "cap": chi, "fidelity": mps_fidelity(torch, bm.tensors(j),
                                                                           m.tensors),
                          "error_bound_batched": eb, "error_bound_plain": ep})
+        log(f"batched: unbatched reference on {len(fids)} MPS done")
         light = [f["fidelity"] for f in fids if f["error_bound_plain"] < 1e-3]
         heavy = [abs(f["error_bound_batched"] - f["error_bound_plain"]) / f["error_bound_plain"]
                  for f in fids if f["error_bound_plain"] >= 1e-3]
@@ -663,6 +671,7 @@ def batched_bench(torch, a, dev):
            "circuit_s": t_circ, "circuit_s_runs": runs, "updates_per_s": n2q / t_circ,
            "max_bond_reached": max(chis),
            "eig_fallbacks": int(bm.eig_fallbacks),
+           "eig_solver_above_order_64": a.batched_eig,
            "shots": S * len(specs), "sample_s": t_samp,
            "shots_per_s": S * len(specs) / t_samp, "sampler": "grouped sweep, HIP-graph replay",
            "unbatched_reference": ref,

@@ -63,6 +63,20 @@ int aq_desc_sizes(int64_t* out);
 int aq_theta(int dtype, int prec, const void* A, const void* B, const double* gate_host,
              void* X, int64_t chiL, int64_t chiM, int64_t chiR, hipStream_t stream);
 
+/* aq_theta for complex64 in BF16x3 with the operands split once, before the
+ * GEMM, into four BF16 planes (re hi / lo, im hi / lo) in caller workspace
+ * `ws` (16-byte aligned, >= aq_theta_planes_bytes(chiL, chiM, chiR) bytes),
+ * k-contiguous and zero-padded to whole tiles, so the MFMA mainloop only
+ * copies them into LDS.  Same contraction, gate and result as aq_theta with
+ * AQ_PREC_BF16X3 (replaces the same reference functions:
+ * src/triton_kernels/mps_complex.py:343-368, as called at
+ * src/atlas_q/adaptive_mps.py:225-247); the planes are valid after the call
+ * and overwritten by the next one. */
+int64_t aq_theta_planes_bytes(int64_t chiL, int64_t chiM, int64_t chiR);
+int aq_theta_planes(const void* A, const void* B, const double* gate_host, void* X,
+                    int64_t chiL, int64_t chiM, int64_t chiR, void* ws, int64_t ws_bytes,
+                    hipStream_t stream);
+
 /* Rank rule: replaces choose_rank_from_sigma (src/atlas_q/truncation.py:21-83)
  * as called at adaptive_mps.py:266/273.  S: m singular values (float32 when
  * s_dtype == AQ_C64, float64 when AQ_C128), descending.  cap: per-bond cap
@@ -190,6 +204,20 @@ int aq_heev_small_orders(int64_t n, const int64_t* orders, const void* A, int64_
 int aq_heevd_batched(int64_t n, void* A, int64_t strideA, double* W, int64_t strideW, double* E,
                      int32_t* info, int64_t batch, hipStream_t stream);
 
+/* One Cholesky QR pass of a tall-skinny complex128 matrix (no reference
+ * counterpart: the orthonormalisation of the randomized SVD's sketch, which
+ * sizes the factorization that replaces linalg_robust.py:17-52 as called at
+ * adaptive_mps.py:249-250): W = Y^H Y + shift_scale trace(W) I, W = L L^H
+ * (blocked by 64, diagonal blocks factored and inverted in LDS), Q = Y L^{-H}.
+ * Y [m, l] row-major (row stride ldy), Q [m, l] (row stride ldq, must not
+ * alias Y), 1 <= l <= min(m, 4096).  ws: complex128 workspace of
+ * aq_cholqr_workspace_elems(m, l) elements.  info (device int): 0, or 1 + the
+ * index of the first pivot that was not positive and finite (then Q is not
+ * orthonormal; the pass still ends with finite numbers). */
+int64_t aq_cholqr_workspace_elems(int64_t m, int64_t l);
+int aq_cholqr(int64_t m, int64_t l, const void* Y, int64_t ldy, void* Q, int64_t ldq,
+              double shift_scale, void* ws, int64_t ws_elems, int* info, hipStream_t stream);
+
 /* Strided-batched complex GEMM with a scaled accumulate (no reference
  * counterpart: the steps of the Gram eigensolver below, which replaces the
  * factorization of adaptive_mps.py:249-250 / linalg_robust.py:17-52 in the
@@ -227,8 +255,11 @@ int aq_bgemm_lower(int64_t M, int64_t K, double alpha, const void* A, int64_t a_
  * j of Zt (complex128 [batch, >= kmax, n], matrix stride sZ) = the unit
  * eigenvector of W[b, j] (twisted inverse iteration; rows kvec[b] .. kmax-1
  * zeroed, so with kvec the padded row count kmax may exceed n as long as
- * every kvec[b] <= n); scratch: aq_tridiag_eigvecs_scratch_doubles()
- * doubles.
+ * every kvec[b] <= n); kept eigenvalues of one unreduced block that chain
+ * within 1e-5 ||T|| of each other are a cluster, whose members after the
+ * first are recomputed by inverse iteration with Gram-Schmidt against the
+ * earlier ones (LAPACK dstein's scheme); scratch:
+ * aq_tridiag_eigvecs_scratch_doubles() doubles.
  * aq_hetrd_vblocks / aq_hetrd_tfactors: the vectors v_i in blocks of nb <=
  * 64 (V [batch, nblk, nb, n], nblk = ceil((n-1)/nb), zero rows past i = n-2)
  * and each block's forward compact-WY factor T' (nb x nb, from the block's

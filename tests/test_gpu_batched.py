@@ -153,11 +153,70 @@ def test_svd_fill_matches_gram_eigenpairs():
         assert float(W[b, r:].abs().max() if r < sz else 0.0) == 0.0
 
 
+def _exact_fidelities(bm, specs, circs):
+    out = []
+    for j, (n, _) in enumerate(specs):
+        psi = SV.run_circuit(n, circs[j])
+        t = bm.tensors(j)
+        res = t[0].reshape(2, -1).to(torch.complex128)
+        for q in range(1, n):
+            T = t[q].to(torch.complex128)
+            res = (res @ T.reshape(T.shape[0], -1)).reshape(-1, T.shape[2])
+        v = res.reshape(-1).cpu().numpy()
+        out.append(abs(np.vdot(psi, v)) ** 2 / np.vdot(v, v).real)
+    return out
+
+
 def test_batched_wide_buckets_exact_against_statevector(monkeypatch):
-    """Orders above the Jacobi limit (rocSOLVER strided-batched buckets of
-    order 128 and 256) without truncation equal the exact state; one bucket
-    row reported as failed goes through the theta-SVD refill and the state
-    is still exact."""
+    """Orders above the Jacobi limit (tridiagonal-solver buckets of order 128
+    and 256) without truncation equal the exact state; one bucket row whose
+    vectors are reported as failing the orthogonality re-check goes through
+    the theta-SVD refill and the state is still exact."""
+    from atlas_q.eig import GramTridiag
+    specs = [(16, 256), (14, 128)]
+    circs = [hea(n, 9, 300 + j) for j, (n, _) in enumerate(specs)]
+    real = GramTridiag.vectors
+    seen = {"calls": 0}
+
+    def one_failure(self, ks):
+        Et, err = real(self, ks)
+        seen["calls"] += 1
+        if seen["calls"] == 1:
+            err = err.clone()
+            err[0] = 1.0
+        return Et, err
+    monkeypatch.setattr(GramTridiag, "vectors", one_failure)
+    bm = BatchedMPS(specs, eps_bond=1e-10, device=DEV, autotuner=_tuner())
+    bm.run_hea(circs)
+    assert seen["calls"] > 0 and bm.eig_fallbacks >= 1
+    assert max(int(bm.dims[j].max()) for j in range(len(specs))) > 64
+    fids = _exact_fidelities(bm, specs, circs)
+    print("fidelities vs state vector", fids)
+    for j, fid in enumerate(fids):
+        assert abs(1 - fid) < 1e-5, (j, fid)
+
+
+def test_batched_hea_tridiag_buckets_no_fallbacks():
+    """The CZ-ladder states' Gram matrices repeat eigenvalues exactly; with
+    buckets of order 128-512 on the tridiagonal solver every bucket row
+    passes the orthogonality re-check (no theta-SVD refill) and the states
+    equal the exact state vector (cutoff 1e-10, caps not binding)."""
+    specs = [(16, 256), (17, 256), (15, 128), (18, 512)]
+    circs = [hea(n, 8, 900 + j) for j, (n, _) in enumerate(specs)]
+    bm = BatchedMPS(specs, eps_bond=1e-10, device=DEV, autotuner=_tuner())
+    bm.run_hea(circs)
+    fids = _exact_fidelities(bm, specs, circs)
+    print("bond max", [int(bm.dims[j].max()) for j in range(len(specs))],
+          "fallbacks", bm.eig_fallbacks, "fidelities", fids)
+    assert max(int(bm.dims[j].max()) for j in range(len(specs))) > 64
+    assert bm.eig_fallbacks == 0
+    for j, fid in enumerate(fids):
+        assert abs(1 - fid) < 1e-5, (j, fid)
+
+
+def test_batched_wide_buckets_rocsolver_comparison_path(monkeypatch):
+    """The rocSOLVER divide-and-conquer buckets (eig_solver="heevd", the
+    comparison point) give the exact state too, a failed row included."""
     import atlas_q.batched as BM
     specs = [(16, 256), (14, 128)]
     circs = [hea(n, 9, 300 + j) for j, (n, _) in enumerate(specs)]
@@ -172,20 +231,49 @@ def test_batched_wide_buckets_exact_against_statevector(monkeypatch):
             ok[0] = False
         return W, V, ok
     monkeypatch.setattr(BM, "heevd_batched", one_failure)
-    bm = BatchedMPS(specs, eps_bond=1e-10, device=DEV, autotuner=_tuner())
+    bm = BatchedMPS(specs, eps_bond=1e-10, device=DEV, autotuner=_tuner(), eig_solver="heevd")
     bm.run_hea(circs)
     assert seen["calls"] > 0 and bm.eig_fallbacks >= 1
-    assert max(int(bm.dims[j].max()) for j in range(len(specs))) > 64
-    for j, (n, _) in enumerate(specs):
-        psi = SV.run_circuit(n, circs[j])
-        t = bm.tensors(j)
-        res = t[0].reshape(2, -1).to(torch.complex128)
-        for q in range(1, n):
-            T = t[q].to(torch.complex128)
-            res = (res @ T.reshape(T.shape[0], -1)).reshape(-1, T.shape[2])
-        out = res.reshape(-1).cpu().numpy()
-        fid = abs(np.vdot(psi, out)) ** 2 / np.vdot(out, out).real
+    for j, fid in enumerate(_exact_fidelities(bm, specs, circs)):
         assert abs(1 - fid) < 1e-5, (j, fid)
+
+
+def test_batched_tridiag_and_rocsolver_buckets_agree():
+    """Truncating brickwork (caps 40-72, buckets of order 128 and 256): the
+    library's tridiagonal buckets and rocSOLVER's divide-and-conquer keep
+    the same ranks and states (fidelity 1 within 1e-6 per MPS), with no
+    fallback in either."""
+    specs = [(12, 40), (13, 50), (14, 72)]
+    depth = 12
+    circs = [brickwork(n, depth, 170 + j) for j, (n, _) in enumerate(specs)]
+    runs = []
+    for solver in ("tridiag", "heevd"):
+        bm = BatchedMPS(specs, eps_bond=1e-8, device=DEV, autotuner=_tuner(), eig_solver=solver)
+        for layer in range(depth):
+            ops = []
+            for j, (n, _) in enumerate(specs):
+                per = len(range(layer % 2, n - 1, 2))
+                start = sum(len(range(l % 2, n - 1, 2)) for l in range(layer))
+                ops.append([(b, U) for b, U in circs[j][start:start + per]])
+            bm.apply_2q_layer(ops)
+        assert bm.eig_fallbacks == 0, solver
+        runs.append(bm)
+    a, b = runs
+    assert a.logs["k"] == b.logs["k"]
+    for j, (n, _) in enumerate(specs):
+        va = _dense(a.tensors(j))
+        vb = _dense(b.tensors(j))
+        fid = abs(np.vdot(va, vb)) ** 2 / (np.vdot(va, va).real * np.vdot(vb, vb).real)
+        print(f"mps {j}: fidelity tridiag vs rocSOLVER buckets {fid:.15f}")
+        assert abs(1 - fid) < 1e-6
+
+
+def _dense(ts):
+    res = ts[0].reshape(2, -1).to(torch.complex128)
+    for T in ts[1:]:
+        T = T.to(torch.complex128)
+        res = (res @ T.reshape(T.shape[0], -1)).reshape(-1, T.shape[2])
+    return res.reshape(-1).cpu().numpy()
 
 
 def test_batched_padded_buckets_match_single_mps_gram():
@@ -226,7 +314,7 @@ def test_batched_padded_buckets_match_single_mps_gram():
 
 
 def test_concurrent_bucket_eigensolves_match_one_stream():
-    """The rocSOLVER size buckets of a sub-layer solved on three worker
+    """The tridiagonal-solver size buckets of a sub-layer solved on three worker
     streams (each waiting for the Gram launch, the caller's stream waiting
     for each of them) give bit-identical states and logs to one stream."""
     specs = [(16, 256), (14, 128), (13, 50)]
@@ -240,7 +328,7 @@ def test_concurrent_bucket_eigensolves_match_one_stream():
         runs.append((bm.arena.clone(), {k: list(v) for k, v in bm.logs.items()},
                      bm.eig_fallbacks, max(int(bm.dims[j].max()) for j in range(len(specs)))))
     (a1, l1, f1, d1), (a3, l3, f3, d3) = runs
-    assert d1 > 64 and d1 == d3          # rocSOLVER buckets were used
+    assert d1 > 64 and d1 == d3          # buckets above the Jacobi limit were used
     assert f1 == f3
     assert l1 == l3
     assert torch.equal(a1, a3)

@@ -46,6 +46,30 @@ def test_theta_matches_oracle(dtype, shape):
     assert _relerr(X, ref) < TOL[dtype]
 
 
+@pytest.mark.parametrize("shape", [(1, 1, 1), (3, 4, 5), (33, 65, 70), (130, 100, 7),
+                                   (64, 96, 64), (256, 200, 300)])
+def test_theta_presplit_planes_equal_inline_split(shape):
+    """aq_theta_planes (operands split once into BF16 planes, zero-padded to
+    whole tiles) returns exactly the bits of aq_theta's BF16x3 path (split
+    while staging each tile): the same splits, products and accumulation
+    order, the padding adding exact zeros."""
+    import ctypes
+    from atlas_q._ffi import AQ_C64, AQ_PREC_BF16X3, check, lib
+    cl, cm, cr = shape
+    g = torch.Generator().manual_seed(cl + 7 * cm + 13 * cr)
+    A = _rand((cl, 2, cm), torch.complex64, g).to(DEV)
+    B = _rand((cm, 2, cr), torch.complex64, g).to(DEV)
+    U = haar_su4(np.random.default_rng(cr))
+    gh = np.ascontiguousarray(K.gate_to_host(U, (4, 4)), np.float64)
+    X1 = torch.empty((2 * cl, 2 * cr), dtype=torch.complex64, device=DEV)
+    check(lib().aq_theta(AQ_C64, AQ_PREC_BF16X3, K._ptr(A), K._ptr(B),
+                         gh.ctypes.data_as(ctypes.c_void_p), K._ptr(X1), cl, cm, cr,
+                         K._stream(A)), "aq_theta")
+    X2 = K.theta(A, B, gh, prec="bf16x3")
+    assert K.THETA_PRESPLIT
+    assert torch.equal(X1, X2), float((X1 - X2).abs().max())
+
+
 @pytest.mark.parametrize("shape", [(16, 32, 24), (64, 96, 64), (128, 128, 128)])
 def test_theta_bf16x3_error_bound(shape):
     cl, cm, cr = shape
