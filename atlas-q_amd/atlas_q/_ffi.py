@@ -75,6 +75,8 @@ SIGNATURES = {
     "aq_last_error": (ctypes.c_char_p, []),
     "aq_theta": (c_int, [c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_i64, c_vp]),
     "aq_theta_planes_bytes": (c_i64, [c_i64, c_i64, c_i64]),
+    "aq_gram_svd_small": (c_int, [c_i64, c_vp, c_i64, c_i64, c_vp, c_vp, c_i64, c_i64, c_i64,
+                                  c_int, ctypes.c_double, c_vp, c_vp]),
     "aq_cholqr_workspace_elems": (c_i64, [c_i64, c_i64]),
     "aq_cholqr": (c_int, [c_i64, c_i64, c_vp, c_i64, c_vp, c_i64, ctypes.c_double, c_vp, c_i64,
                           c_vp, c_vp]),

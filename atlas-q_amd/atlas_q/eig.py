@@ -240,21 +240,48 @@ class GramTridiag:
         return Et, err
 
 
+def gram_svd_small(G: torch.Tensor, max_sweeps: int = 30, tol: float = 1e-15):
+    """One-sided Jacobi SVD of the Cholesky factor of Hermitian positive
+    semidefinite complex128 matrices G [batch, n, n] or [n, n] (n <= 64):
+    (eigenvalues desc, eigenvectors as columns, FP64 re-check err [batch])."""
+    from . import kernels as K
+    from ._ffi import check, lib
+    single = G.dim() == 2
+    if single:
+        G = G.unsqueeze(0)
+    G = G.to(torch.complex128).contiguous()
+    bsz, n, _ = G.shape
+    W = torch.empty((bsz, n), dtype=torch.float64, device=G.device)
+    U = torch.empty((bsz, n, n), dtype=torch.complex128, device=G.device)
+    err = torch.empty(bsz, dtype=torch.float64, device=G.device)
+    check(lib().aq_gram_svd_small(n, K._ptr(G), n, n * n, K._ptr(W), K._ptr(U), n, n * n, bsz,
+                                  max_sweeps, tol, K._ptr(err), K._stream(G)),
+          "aq_gram_svd_small")
+    return (W[0], U[0], err[0]) if single else (W, U, err)
+
+
 class TopEig:
     """Eigenvalues (descending, ``values``) of one Hermitian complex128 matrix
     G [n, n] and, on request, the eigenvectors of its k largest
-    (``vectors(k)``: columns [n, k]).  Orders <= 64: the Jacobi kernel (all
-    pairs in one launch).  Larger: the tridiagonal path (reduction and
-    bisection now, inverse iteration and back-transformation for the k
-    vectors later); vectors that fail the FP64 orthogonality re-check are
-    taken from ``eigh_desc`` instead (``fallbacks`` counts them)."""
+    (``vectors(k)``: columns [n, k]).  Orders <= 64: the two-sided Jacobi
+    kernel, or with ``small="svd"`` (a positive semidefinite Gram core) the
+    one-sided Jacobi SVD of its Cholesky factor.  Larger: the tridiagonal
+    path (reduction and bisection now, inverse iteration and
+    back-transformation for the k vectors later).  Vectors that fail the
+    FP64 orthogonality re-check (or lie past the one-sided factor's
+    numerical rank) are taken from the two-sided Jacobi kernel / ``eigh_desc``
+    instead (``fallbacks`` counts them)."""
 
     fallbacks = 0
 
-    def __init__(self, G: torch.Tensor):
+    def __init__(self, G: torch.Tensor, small: str = "jacobi"):
         self.G = G
         n = G.shape[-1]
-        if n <= JACOBI_MAX:
+        self._err = None
+        if n <= JACOBI_MAX and small == "svd":
+            self.values, self._V, self._err = gram_svd_small(G)
+            self._tri = None
+        elif n <= JACOBI_MAX:
             from .rank_mlp import heev_small
             self.values, self._V = heev_small(G)
             self._tri = None
@@ -266,6 +293,13 @@ class TopEig:
 
     def vectors(self, k: int) -> torch.Tensor:
         k = int(k)
+        if self._err is not None:
+            host = torch.stack([self._err, self.values[k - 1]]).cpu().tolist()
+            self._err = None
+            if not (host[0] <= ORTH_TOL and host[1] > 0.0):
+                from .rank_mlp import heev_small
+                TopEig.fallbacks += 1
+                _, self._V = heev_small(self.G)
         if self._V is not None:
             return self._V[:, :k]
         Et, err = self._tri.vectors([k])

@@ -7,9 +7,10 @@ For X [m, n] (complex64) and a width l = k_guess + p, p = max(16, k_guess):
   Q = orth(Y)                      shifted Cholesky-QR, twice, in FP64
                                    (the library's aq_cholqr)
   B = Q^H X        [l, n]          FP64 MFMA GEMM
-  B B^H = U_B diag(lam) U_B^H      FP64 Hermitian core (Jacobi kernel up to
-                                   order 64, else the tridiagonal path; only
-                                   the kept k vectors are formed)
+  B B^H = U_B diag(lam) U_B^H      FP64 Hermitian core (up to order 64 the
+                                   one-sided Jacobi SVD of its Cholesky
+                                   factor, else the tridiagonal path; only the
+                                   kept k vectors are formed)
   tail = ||X||_F^2 - sum(lam)      FP64: weight outside range(Q)
 The captured eigenvalues bound the true squared singular values:
 sigma_i^2 >= lam_i (Cauchy interlacing for the projection Q^H X) and
@@ -193,7 +194,9 @@ def try_rsvd(mps, i, X, cap, budget_ok, local, cl, cr, st, k_guess):
     with RL.timed("rsvd_project_fp64", 8.0 * l * m * n, "fp64_mfma"):
         Bc = K.cgemm(Q.T, X64, conj_a=True)                       # [l, n]
     G = K.cgemm(Bc, Bc.T, conj_b=True)                           # [l, l]
-    core = TopEig(G)            # eigenvalues now, the kept vectors after the rank rule
+    # eigenvalues now, the kept vectors after the rank rule; orders <= 64 by
+    # the one-sided Jacobi SVD of the core's Cholesky factor
+    core = TopEig(G, small="svd")
     lam = torch.clamp(core.values, min=0.0)
     # one device-to-host transfer: captured spectrum, ||X||^2, factorization codes
     host = torch.cat([lam, torch.vdot(X64.reshape(-1), X64.reshape(-1)).real.view(1),

@@ -325,9 +325,11 @@ __global__ __launch_bounds__(kTriThreads) void tri_cluster_kernel(
     if (e[i] == 0.0) atomicAdd(&nsplit, 1);
   __syncthreads();
   const double* Wb = W + b * sW;
-  double* LF = scratch + b * sScr;                  // [n][kmax]
-  double* DF = LF + (int64_t)n * kmax;
-  double* X = DF + (int64_t)n * kmax;
+  // three disjoint [n][kmax] blocks (restrict: the loads of the solves'
+  // next rows may be issued ahead of the current row's store)
+  double* __restrict__ LF = scratch + b * sScr;
+  double* __restrict__ DF = LF + (int64_t)n * kmax;
+  double* __restrict__ X = DF + (int64_t)n * kmax;
   int* bs = (int*)(X + (int64_t)n * kmax);
   int* bt = bs + kmax;
   int* cl = bt + kmax;

@@ -204,6 +204,23 @@ int aq_heev_small_orders(int64_t n, const int64_t* orders, const void* A, int64_
 int aq_heevd_batched(int64_t n, void* A, int64_t strideA, double* W, int64_t strideW, double* E,
                      int32_t* info, int64_t batch, hipStream_t stream);
 
+/* One-sided (Hestenes) Jacobi SVD of small Gram cores, FP64, orders
+ * n <= 64 (no reference counterpart: the factorization of the randomized
+ * SVD's projected core, part of the method that replaces
+ * linalg_robust.py:17-52 as called at adaptive_mps.py:249-250).  G:
+ * complex128 row-major Hermitian positive semidefinite matrices (lower
+ * triangle read) at G + b*strideG.  G = L L^H (Cholesky in LDS, stopped at
+ * the first pivot below n eps max G_ii: the rest of L is zero), then the
+ * columns of L are rotated until orthogonal, L J = U Sigma, with
+ * wavefront-level reductions for the inner products.  Outputs: W[b, :]
+ * = Sigma^2 descending (the eigenvalues of G), U [b, n, n] row-major with the
+ * corresponding unit eigenvectors as columns (zero columns past the numerical
+ * rank), err[b] = max |U^H U - I| over the non-zero columns (FP64 re-check).
+ * Sweeps stop when every pair has |a_p^H a_q| <= tol ||a_p|| ||a_q||. */
+int aq_gram_svd_small(int64_t n, const void* G, int64_t ldg, int64_t strideG, double* W, void* U,
+                      int64_t ldu, int64_t strideU, int64_t batch, int max_sweeps, double tol,
+                      double* err, hipStream_t stream);
+
 /* One Cholesky QR pass of a tall-skinny complex128 matrix (no reference
  * counterpart: the orthonormalisation of the randomized SVD's sketch, which
  * sizes the factorization that replaces linalg_robust.py:17-52 as called at

@@ -11,6 +11,12 @@
   eigenvalues against numpy's eigvalsh to 1e-12 relative to the largest,
   residuals |G v - lambda v| to 1e-11, orthonormal columns, at orders 16
   (Jacobi kernel), 65, 100, 200 and on a rank-deficient Gram matrix.
+* ``aq_gram_svd_small`` (one-sided Jacobi SVD of the Cholesky factor of a
+  Gram core, orders <= 64, wavefront reductions): eigenvalues against
+  numpy's eigvalsh to 1e-12 of the largest, residuals 1e-11, the in-kernel
+  FP64 re-check <= 1e-10 and equal to the host's |U^H U - I|; a
+  rank-deficient core returns zero columns past its numerical rank (and
+  TopEig then takes the two-sided kernel for vectors there).
 * The Gram method's eigensolver-failure branch takes the plain method's SVD
   cascade with its complex128 promotion (forced by a failing eigensolver).
 """
@@ -159,3 +165,48 @@ def test_gram_eigensolver_failure_takes_plain_cascade_with_promotion(monkeypatch
     assert any(t.dtype == torch.complex128 for t in got.tensors)
     for bstr in (0, 5, 2 ** n - 1):
         assert abs(got.get_amplitude(bstr) - ref.get_amplitude(bstr)) < 1e-6
+
+
+@pytest.mark.parametrize("n", [1, 2, 17, 33, 64])
+def test_gram_svd_small_matches_lapack(n):
+    from atlas_q.eig import ORTH_TOL, gram_svd_small
+    rng = np.random.default_rng(100 + n)
+    Gs = []
+    for decay in (None, 8.0):
+        X = _cz(rng, (n, 3 * n + 1))
+        if decay is not None:
+            X = X * np.exp(-np.arange(n) / decay)[:, None]
+        Gs.append(X @ X.conj().T)
+    W, U, err = gram_svd_small(torch.from_numpy(np.stack(Gs)).to(DEV))
+    W, U, err = W.cpu().numpy(), U.cpu().numpy(), err.cpu().numpy()
+    for b, G in enumerate(Gs):
+        ref = np.sort(np.linalg.eigvalsh(G))[::-1]
+        ew = np.abs(W[b] - ref).max() / ref[0]
+        res = np.abs(G @ U[b] - U[b] * W[b][None, :]).max() / ref[0]
+        orth = np.abs(U[b].conj().T @ U[b] - np.eye(n)).max()
+        print(f"n={n}[{b}]: eig {ew:.1e}, residual {res:.1e}, orth {orth:.1e} (re-check {err[b]:.1e})")
+        assert ew < 1e-12 and res < 1e-11
+        assert err[b] <= ORTH_TOL and orth <= ORTH_TOL
+        assert abs(orth - err[b]) < 1e-13
+
+
+def test_gram_svd_small_rank_deficient_core():
+    from atlas_q.eig import TopEig, gram_svd_small
+    rng = np.random.default_rng(11)
+    n, r = 48, 20
+    X = _cz(rng, (n, r)) @ _cz(rng, (r, 100))
+    G = X @ X.conj().T
+    W, U, err = gram_svd_small(torch.from_numpy(G).to(DEV))
+    W, U = W.cpu().numpy(), U.cpu().numpy()
+    ref = np.sort(np.linalg.eigvalsh(G))[::-1]
+    assert np.abs(W - ref).max() < 1e-12 * ref[0]
+    assert float(err) <= 1e-10
+    Ur = U[:, :r]
+    assert np.abs(G @ Ur - Ur * W[None, :r]).max() < 1e-11 * ref[0]
+    assert np.abs(Ur.conj().T @ Ur - np.eye(r)).max() < 1e-10
+    # vectors past the rank: TopEig takes the two-sided kernel
+    before = TopEig.fallbacks
+    te = TopEig(torch.from_numpy(G).to(DEV), small="svd")
+    V = te.vectors(n).cpu().numpy()
+    assert TopEig.fallbacks == before + 1
+    assert np.abs(V.conj().T @ V - np.eye(n)).max() < 1e-10
