@@ -43,6 +43,10 @@ COPY_DESC = _np.dtype([("src", "u8"), ("dst", "u8"), ("row_scale", "u8"), ("col_
                        ("rows", "i8"), ("cols", "i8"), ("s0", "i8"), ("s1", "i8"),
                        ("conj", "i4"), ("pad", "i4")])
 AQ_GROUPED_GEMM, AQ_GROUPED_THETA, AQ_GROUPED_HERK = 0, 1, 2
+LAYER_GATE = _np.dtype([("siteA", "u8"), ("siteB", "u8"), ("X", "u8"), ("X64", "u8"), ("G", "u8"),
+                        ("V", "u8"), ("S64", "u8"), ("Sinv", "u8"), ("P", "u8"), ("dims", "u8"),
+                        ("bond", "i8"), ("sz", "i8"), ("left", "i8"), ("vfmt", "i8"),
+                        ("kslot", "i8"), ("kbound", "i8"), ("gate", "f8", (32,))])
 
 
 class RankResult(ctypes.Structure):
@@ -77,6 +81,13 @@ SIGNATURES = {
     "aq_theta_planes_bytes": (c_i64, [c_i64, c_i64, c_i64]),
     "aq_gram_svd_small": (c_int, [c_i64, c_vp, c_i64, c_i64, c_vp, c_vp, c_i64, c_i64, c_i64,
                                   c_int, ctypes.c_double, c_vp, c_vp]),
+    "aq_hetrd_max_groups": (c_int, [c_int]),
+    "aq_layer_gate_size": (c_i64, []),
+    "aq_layer_pre": (c_int, [c_vp, c_int, c_vp, c_vp, c_vp, c_vp]),
+    "aq_layer_pad": (c_int, [c_vp, c_i64, c_i64, c_vp, c_vp, c_vp]),
+    "aq_layer_post": (c_int, [c_vp, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
+                              c_vp]),
+    "aq_layer_1q": (c_int, [c_vp, c_vp, c_vp, c_int, c_vp]),
     "aq_cholqr_workspace_elems": (c_i64, [c_i64, c_i64]),
     "aq_cholqr": (c_int, [c_i64, c_i64, c_vp, c_i64, c_vp, c_i64, ctypes.c_double, c_vp, c_i64,
                           c_vp, c_vp]),
@@ -190,6 +201,8 @@ def check_layouts() -> None:
             ctypes.sizeof(DenseLayer)]
     if list(out) != want:
         raise RuntimeError(f"descriptor layout mismatch: library {list(out)} vs binding {want}")
+    if int(lib().aq_layer_gate_size()) != LAYER_GATE.itemsize:
+        raise RuntimeError("aq_layer_gate_t layout mismatch")
 
 
 def check(status: int, what: str) -> None:

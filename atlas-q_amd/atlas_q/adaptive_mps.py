@@ -250,7 +250,7 @@ class AdaptiveMPS(MatrixProductStatePyTorch):
         else:
             outs = [prep(i, U) for i, U in gates]
         jobs = [j for j, (r, _) in enumerate(outs) if isinstance(r, fastsvd.GramJob)]
-        for j, res in zip(jobs, fastsvd.finish_gram(self, [outs[j][0] for j in jobs])):
+        for j, res in zip(jobs, fastsvd.finish_gram(self, [outs[j][0] for j in jobs], workers)):
             outs[j][0] = res
         return [tuple(r) + (cm,) for r, cm in outs]
 

@@ -178,20 +178,41 @@ class GramTridiag:
               "aq_tridiag_eigvals")
         return self.W
 
-    def vectors(self, ks):
+    @staticmethod
+    def vectors_buffer(B: int, n: int, kpad: int, device) -> torch.Tensor:
+        """Zeroed storage for ``vectors(out=...)``: Et [B, kpad, n] plus the
+        tail the orthogonality re-check's last tile reads."""
+        return torch.zeros(B * kpad * n + ORTH_TILE * n, dtype=torch.complex128, device=device)
+
+    def vectors(self, ks=None, kvec: torch.Tensor = None, kpad: int = None,
+                out: torch.Tensor = None):
         """(Et [B, kpad, n] complex128, err [B] float64 on the device): row j <
         ks[b] of Et[b] is the conjugate transpose of the eigenvector of the
         j-th largest eigenvalue (rows >= ks[b] are zero); err[b] the largest
-        |Et Et^H - I| entry over the diagonal tiles of the first ks[b] rows."""
+        |Et Et^H - I| entry over the diagonal tiles of the first ks[b] rows.
+        The kept ranks come from the host list ``ks`` or, with no host round
+        trip, from ``kvec`` (int32 [B] on the device) with the row count
+        ``kpad`` (a multiple of 64 and at least every kvec[b]); ``out`` is
+        optional storage from ``vectors_buffer``."""
         from . import kernels as K
         from ._ffi import check, lib
         B, n, dev = self.B, self.n, self.G.device
         st = K._stream(self.G)
         L = lib()
-        kmax = max(1, max(int(k) for k in ks))
-        kpad = -(-kmax // ORTH_TILE) * ORTH_TILE
-        kv = torch.tensor([int(k) for k in ks], dtype=torch.int32).to(dev)
-        flat = torch.zeros(B * kpad * n + ORTH_TILE * n, dtype=torch.complex128, device=dev)
+        if kvec is None:
+            kmax = max(1, max(int(k) for k in ks))
+            kpad = -(-kmax // ORTH_TILE) * ORTH_TILE
+            kv = torch.tensor([int(k) for k in ks], dtype=torch.int32).to(dev)
+        else:
+            if kpad is None or kpad % ORTH_TILE or kvec.dtype != torch.int32 or kvec.numel() != B:
+                raise ValueError("vectors: kvec needs int32 [B] and kpad a multiple of 64")
+            kv = kvec
+        if out is None:
+            flat = self.vectors_buffer(B, n, kpad, dev)
+        else:
+            if out.numel() != B * kpad * n + ORTH_TILE * n or out.dtype != torch.complex128:
+                raise ValueError("vectors: out must come from vectors_buffer(B, n, kpad)")
+            flat = out
         Et = flat[:B * kpad * n].view(B, kpad, n)
         nsc = int(L.aq_tridiag_eigvecs_scratch_doubles(n, kpad, B))
         sc = torch.empty(nsc, dtype=torch.float64, device=dev)

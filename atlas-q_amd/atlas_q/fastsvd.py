@@ -435,7 +435,7 @@ def _gram_group_tridiag(mps, jobs, st):
     return out
 
 
-def finish_gram(mps, jobs):
+def finish_gram(mps, jobs, workers: int = 1):
     """Exact Gram factorization of several updates.  Complex64 updates of one
     order above 64 with m <= n are factored together (``_gram_group_tridiag``:
     one launch per step for the whole group); every other update gets its own
@@ -443,7 +443,10 @@ def finish_gram(mps, jobs):
     tridiagonal path with the kept vectors only).  If the eigensolver fails
     (vectors failing the orthogonality re-check and rocSOLVER's jittered
     retries after them), the update takes the plain method's SVD cascade,
-    precision promotion included."""
+    precision promotion included.  With ``workers`` > 1 the updates factored
+    one by one run concurrently on that many host threads, each with its own
+    HIP stream (an order above 64 is a chain of small per-column launches,
+    which overlap across streams)."""
     from .eig import JACOBI_MAX
     st = _factor_stats(mps)
     orders = {}
@@ -460,13 +463,7 @@ def finish_gram(mps, jobs):
             for i, r in zip(grouped, res):
                 if r is not None:
                     done[i] = r
-    outs = []
-    for idx, j in enumerate(jobs):
-        if idx in done:
-            st.bump("gram")
-            outs.append(done[idx])
-            _harvest(mps, j.feats, done[idx][2])
-            continue
+    def one(j):
         try:
             G, left, X64 = _gram_matrix(j.X)
             te = TopEig(G)
@@ -478,10 +475,40 @@ def finish_gram(mps, jobs):
             # the FP64 eigensolver failed: the plain method's SVD cascade
             res = plain_factor(mps, j.X, j.cap, j.budget_ok, j.local, j.cl, j.cr)
             st.bump("plain")
-            outs.append(res)
-            _harvest(mps, j.feats, res[2])
-            continue
+            return res
         st.bump("gram")
-        outs.append((A_new, B_new, k, e, s, c, "gram_fp64", dt))
-        _harvest(mps, j.feats, k)
+        return (A_new, B_new, k, e, s, c, "gram_fp64", dt)
+
+    rest = [idx for idx in range(len(jobs)) if idx not in done]
+    if workers > 1 and len(rest) > 1:
+        from .adaptive_mps import _thread_stream, _worker_pool
+        main = torch.cuda.current_stream(jobs[0].X.device)
+        ready = main.record_event()
+        dev = jobs[0].X.device
+
+        def job(idx):
+            stw = _thread_stream(dev)
+            with torch.cuda.stream(stw):
+                stw.wait_event(ready)
+                res = one(jobs[idx])
+                fin = stw.record_event()
+            return res, fin
+
+        pool = _worker_pool(workers)
+        futs = {idx: pool.submit(job, idx) for idx in rest}
+        for idx, f in futs.items():
+            res, fin = f.result()
+            main.wait_event(fin)
+            for t in res[0:2]:          # produced on a worker stream, used on this one
+                t.record_stream(main)
+            done[idx] = res
+    else:
+        for idx in rest:
+            done[idx] = one(jobs[idx])
+    outs = []
+    for idx, j in enumerate(jobs):
+        if idx not in rest:             # factored by the grouped path
+            st.bump("gram")
+        outs.append(done[idx])
+        _harvest(mps, j.feats, done[idx][2])
     return outs

@@ -68,6 +68,9 @@ __global__ __launch_bounds__(kThreads, 2) void grouped_gemm_kernel(const aq_gemm
       m0 = (local / tn) * kBM;
       j0 = (local % tn) * kBJ;
     }
+    // a tile past the problem (device-planned problems are smaller than the
+    // bounds their tile counts come from): nothing to compute or store
+    if (m0 >= g.M || (paired ? j0 >= half : 2 * j0 >= g.N)) continue;
     AStrided<T> aop{(const c2_t<T>*)g.A, g.a_s0, g.a_s1, g.M, g.K, g.conj_a};
     BStrided<T> bop{(const c2_t<T>*)g.B, g.b_s0, g.b_s1, g.K, g.N, g.conj_b};
     ColMap cm{paired, half};
@@ -147,6 +150,7 @@ __global__ void grouped_1q_kernel(const aq_site1q_desc_t* __restrict__ d,
     const int p = find_owner(prefix, P, e);
     const aq_site1q_desc_t g = d[p];
     const int64_t idx = e - prefix[p];
+    if (idx >= g.chiL * g.chiR) continue;   // a device-planned site below its bound
     const int64_t a = idx / g.chiR, b = idx % g.chiR;
     const float2* in = (const float2*)g.T_in;
     float2* out = (float2*)g.T_out;
@@ -206,6 +210,7 @@ __global__ void grouped_copy_kernel(const aq_copy_desc_t* __restrict__ d,
     const int p = find_owner(prefix, P, e);
     const aq_copy_desc_t g = d[p];
     const int64_t idx = e - prefix[p];
+    if (idx >= g.rows * g.cols) continue;   // a device-planned core below its bound
     const int64_t r = idx / g.cols, c = idx % g.cols;
     double2 v = ((const double2*)g.src)[r * g.s0 + c * g.s1];
     if (g.conj) v.y = -v.y;

@@ -539,6 +539,7 @@ namespace aq {
 constexpr int kHetrdStreams = 2;
 static hipStream_t g_hetrd_streams[64][kHetrdStreams];
 static std::mutex g_hetrd_lock;
+static int g_hetrd_max_groups = kHetrdStreams;
 
 static int hetrd_streams(hipStream_t* out) {
   int dev = 0;
@@ -585,7 +586,7 @@ extern "C" int aq_hetrd_own(int64_t n, int64_t nb, void* Av, int64_t strideA, do
   // streaming matrix-vector launch.
   // (from two matrices on: at small batches every launch is latency-bound,
   // and the second group's chain fills the first one's gaps)
-  const int ngroups = batch >= 2 ? kHetrdStreams : 1;
+  const int ngroups = batch >= 2 ? g_hetrd_max_groups : 1;
   hipStream_t ws_streams[kHetrdStreams];
   HetrdGroup g[kHetrdStreams];
   if (ngroups > 1) {
@@ -686,4 +687,14 @@ extern "C" int aq_hetrd_own(int64_t n, int64_t nb, void* Av, int64_t strideA, do
     AQ_HIP(hipEventDestroy(ready));
   }
   return AQ_OK;
+}
+
+// Stream groups of the following aq_hetrd_own calls (process-wide): 1 keeps
+// the reduction on the caller's stream (for HIP graph capture, where the
+// worker streams would have to join the capture), 2 (the default) overlaps
+// two groups' column chains.  Returns the previous value.
+extern "C" int aq_hetrd_max_groups(int groups) {
+  const int old = g_hetrd_max_groups;
+  if (groups >= 1 && groups <= kHetrdStreams) g_hetrd_max_groups = groups;
+  return old;
 }

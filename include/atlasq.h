@@ -289,6 +289,11 @@ int aq_bgemm_lower(int64_t M, int64_t K, double alpha, const void* A, int64_t a_
  * transpose of rows 64t .. 64t+127), restricted to indices < kvec[b]. */
 int aq_hetrd_batched(int64_t n, void* A, int64_t strideA, double* D, double* E, void* tau,
                      int64_t batch, hipStream_t stream);
+/* Stream groups of the following aq_hetrd_own calls (process-wide, 1 or 2):
+ * 1 keeps the whole reduction on the caller's stream (HIP graph capture); 2,
+ * the default, overlaps two groups' column chains on worker streams.
+ * Returns the previous value. */
+int aq_hetrd_max_groups(int groups);
 int aq_tridiag_eigvals(int64_t n, const double* D, int64_t sD, const double* E, int64_t sE,
                        double* W, int64_t sW, int64_t nval, int64_t batch, hipStream_t stream);
 int64_t aq_tridiag_eigvecs_scratch_doubles(int64_t n, int64_t kmax, int64_t batch);
@@ -470,6 +475,59 @@ typedef struct {
 } aq_copy_desc_t;
 int aq_grouped_copy(const aq_copy_desc_t* descs, const int64_t* prefix, int count,
                     int64_t total_elems, int grid, hipStream_t stream);
+
+/* ---------------------------------------------------------------- device-planned layers
+ * A batched two-qubit sub-layer with its bond dimensions on the device (no
+ * reference counterpart: the grouped form of adaptive_mps.py:182-308 for many
+ * MPS, as BatchedMPS runs it, without host round trips so that a whole
+ * circuit can be captured in a HIP graph).  The host fixes every buffer,
+ * bucket and launch shape from upper bounds of the bond dimensions; these
+ * entry points write the size-dependent descriptor fields from the actual
+ * dimensions (dims, int64 per bond of the gate's MPS, updated in place). */
+typedef struct {
+  const void* siteA;      /* complex64 slot of site `bond` */
+  const void* siteB;      /* complex64 slot of site `bond` + 1 */
+  void* X;                /* theta (complex64, bound-sized slot) */
+  void* X64;              /* its complex128 copy */
+  void* G;                /* Gram matrix: row of a size bucket (complex128, sz x sz) */
+  const void* V;          /* eigenvectors: vfmt 0 columns of a row-major sz x sz block,
+                             vfmt 1 conjugate rows Et (kpad x sz) */
+  const double* S64;      /* square roots of the row's eigenvalues, float64 [sz] */
+  const double* Sinv;     /* their reciprocals (0 for 0) */
+  void* P;                /* projection output (complex128, bound-sized slot) */
+  int64_t* dims;          /* the MPS's bond dimensions (device) */
+  int64_t bond;
+  int64_t sz;             /* bucket order */
+  int64_t left;           /* 1: Gram X X^H, 0: X^H X (fixed by the bounds) */
+  int64_t vfmt;
+  int64_t kslot;          /* index into the tridiagonal buckets' kept-rank array, or -1 */
+  int64_t kbound;         /* planned bound of the kept rank (buffers are sized by it) */
+  double gate[32];        /* U[4][4] interleaved re/im, "U" convention */
+} aq_layer_gate_t;
+
+int64_t aq_layer_gate_size(void);   /* sizeof(aq_layer_gate_t), for bindings */
+
+/* Theta descriptors (AQ_GROUPED_THETA, gate in the epilogue), Gram
+ * descriptors (into G, row stride sz) and each Gram matrix's order. */
+int aq_layer_pre(const aq_layer_gate_t* gates, int count, aq_gemm_desc_t* theta,
+                 aq_gemm_desc_t* gram, int64_t* order, hipStream_t stream);
+/* Bucket rows b < rows of T (complex128 [rows, sz, sz]): entries with row or
+ * column >= order[row_gate[b]] set to zero, the diagonal there to
+ * -d (1 + i/sz), d the largest diagonal entry of the leading block. */
+int aq_layer_pad(void* T, int64_t sz, int64_t rows, const int64_t* row_gate,
+                 const int64_t* order, hipStream_t stream);
+/* From the rank results rr[g] (aq_rank_select_batched): the projection
+ * descriptors (Vh_k = U_k^H X or U_k S_k = X V_k, into P), the rebuild
+ * descriptors (copies[g] site `bond`, copies[count + g] site `bond` + 1, as
+ * BatchedMPS), dims[bond + 1] = k, kvec[kslot] = k, and (nullable) the
+ * logs of k, eps_local and entropy.  A rank above the gate's kbound is
+ * lowered to it and counted in *overflow (device int64). */
+int aq_layer_post(const aq_layer_gate_t* gates, int count, const aq_rank_result_t* rr,
+                  aq_gemm_desc_t* proj, aq_copy_desc_t* copies, int32_t* kvec, int64_t* log_k,
+                  double* log_eps, double* log_ent, int64_t* overflow, hipStream_t stream);
+/* Single-qubit descriptors: chiL = dims[g][site[g]], chiR = dims[g][site[g] + 1]. */
+int aq_layer_1q(aq_site1q_desc_t* descs, int64_t* const* dims, const int64_t* site, int count,
+                hipStream_t stream);
 
 /* ---------------------------------------------------------------- tn_core
  * The reference's second two-site update, tools_qih/tn_core.py:mps_apply_2q
