@@ -43,6 +43,7 @@
 namespace aq {
 
 constexpr int kTriThreads = 256;
+constexpr int kClThreads = 1024;   // cluster pass: one thread per member, 16 waves for the projections
 constexpr int kTriMaxN = 4096;  // d, e^2 and e of one matrix in LDS (96 KB at n = 4096)
 
 __device__ __forceinline__ double guard_pivot(double q, double pivmin) {
@@ -69,7 +70,7 @@ struct TriBounds {
 // when `split`), and reduces the Gershgorin bounds over the work-group.
 __device__ TriBounds load_tridiag(const double* __restrict__ D, const double* __restrict__ E,
                                   int n, double* d, double* e2, double* e, bool split) {
-  __shared__ double red[3][kTriThreads / 64];
+  __shared__ double red[3][kClThreads / 64];     // up to the cluster kernel's 16 waves
   double lo = INFINITY, hi = -INFINITY, emax = 0.0;
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
     const double di = D[i];
@@ -270,13 +271,21 @@ __global__ __launch_bounds__(kTriThreads) void tri_invit_kernel(
 // in one block the same vector.  Kept eigenvalues of one unreduced block
 // that chain within gtol = kClusterRel ||T|| of each other form a cluster;
 // every cluster of two or more is recomputed as a block (LAPACK dstein's
-// inverse iteration, its members at once): one thread per member factors
-// T_block - lambda_r (guarded L D L^T) and runs three solves from a fixed
-// pseudo-random start, normalising its own vector; the work-group then
-// orthonormalises the block by modified Gram-Schmidt, twice (norms as
-// work-group reductions, one thread per later column for the projections).
-// Clusters are taken one after another by the matrix's work-group.  The
-// columns live in scratch as [n][kmax] (one coalesced row per step).
+// inverse iteration, its members at once): one thread per member -- the
+// members of all clusters together -- factors T_block - lambda_r (guarded
+// L D L^T) and runs three solves from a fixed pseudo-random start,
+// normalising its own vector, which is written to Z's row.  The clusters'
+// rows are then orthonormalised by multi-work-group passes over Z
+// (big_project_kernel, big_block_kernel), twice: every clustered row is
+// cleared of the vectors of its unreduced block that are in no cluster,
+// then the clustered rows are orthonormalised in steps of kBigR rows in
+// eigenvalue order (Cholesky QR of the step's rows, modified Gram-Schmidt
+// where they are too close to dependent for it, then their projection out
+// of every later clustered row of the block, a work-group per kBigR target
+// rows).  Whole-spectrum chains -- the near-null tail of a Gram matrix,
+// hundreds of members long, where the guarded factorisations are least
+// accurate -- cost O(m^2 n) there, spread over the GPU.  The factors and
+// vectors of the solves live in scratch as [n][kmax] (member j in column j).
 constexpr double kClusterRel = 1e-5;
 
 __device__ __forceinline__ double start_value(int j, int i) {
@@ -303,18 +312,17 @@ __device__ double wg_sum(double v, double* red) {
 // reciprocal pivots, vectors) and ints (block start / end, cluster id, next
 // member, roots, member list per cluster, chain tail per block start).
 __host__ __device__ inline int64_t cluster_scratch_doubles(int64_t n, int64_t kmax) {
-  const int64_t ints = 6 * kmax + n + 8;
+  const int64_t ints = 7 * kmax + n + 8;
   return 3 * n * kmax + (ints + 1) / 2 + 2;
 }
 
 template <int NMAX>
-__global__ __launch_bounds__(kTriThreads) void tri_cluster_kernel(
+__global__ __launch_bounds__(kClThreads) void tri_cluster_kernel(
     int n, const double* __restrict__ D, int64_t sD, const double* __restrict__ E, int64_t sE,
     const double* __restrict__ W, int64_t sW, const int32_t* __restrict__ kvec, int kmax,
     double2* __restrict__ Z, int64_t sZ, double* __restrict__ scratch, int64_t sScr) {
   __shared__ double d[NMAX], e2[NMAX], e[NMAX];
-  __shared__ double red[kTriThreads / 64];
-  __shared__ int nsplit, nroots, msize;
+  __shared__ int nsplit, nroots;
   const int64_t b = blockIdx.x;
   const TriBounds tb = load_tridiag(D + b * sD, E + b * sE, n, d, e2, e, true);
   const int kb = kvec ? min((int)kvec[b], kmax) : kmax;
@@ -337,6 +345,7 @@ __global__ __launch_bounds__(kTriThreads) void tri_cluster_kernel(
   int* roots = nx + kmax;
   int* mem = roots + kmax;
   int* last = mem + kmax;                            // [n]
+  int* bigroot = last + n;                           // [kmax]: root of a deferred cluster, or -1
   const int64_t ld = kmax;
   const double delta = 8.0 * DBL_EPSILON * tb.tnorm + 8.0 * tb.pivmin;
   const double gabs = kClusterRel * tb.tnorm;
@@ -349,6 +358,7 @@ __global__ __launch_bounds__(kTriThreads) void tri_cluster_kernel(
     nx[j] = -1;
   }
   for (int i = threadIdx.x; i < n; i += blockDim.x) last[i] = -1;
+  for (int j = threadIdx.x; j < kmax; j += blockDim.x) bigroot[j] = -1;
   __threadfence_block();
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -370,90 +380,286 @@ __global__ __launch_bounds__(kTriThreads) void tri_cluster_kernel(
   __threadfence_block();
   __syncthreads();
   double2* Zb = Z + b * sZ;
-  for (int q = 0; q < nroots; ++q) {
-    const int root = roots[q];
-    const int s = bs[root], t = bt[root];
-    if (threadIdx.x == 0) {
-      int m = 0;
-      for (int j = root; j >= 0; j = nx[j]) mem[m++] = j;
-      msize = m;
-    }
-    __threadfence_block();
-    __syncthreads();
-    const int m = msize;
-    // inverse iteration, one thread per member
-    for (int r = threadIdx.x; r < m; r += blockDim.x) {
-      const int j = mem[r];
-      const double lam = Wb[j];
-      double dp = d[s] - lam;
+  // every member of every cluster at once, one thread per member, its vector
+  // in column j of the scratch blocks
+  for (int j = threadIdx.x; j < kb; j += blockDim.x) {
+    if (cl[j] == j && nx[j] < 0) continue;           // not in a cluster
+    const int s = bs[j], t = bt[j];
+    const double lam = Wb[j];
+    double dp = d[s] - lam;
+    if (fabs(dp) < pfloor) dp = dp < 0.0 ? -pfloor : pfloor;
+    DF[(int64_t)s * ld + j] = 1.0 / dp;
+    for (int i = s; i < t; ++i) {
+      const double l = e[i] / dp;
+      LF[(int64_t)i * ld + j] = l;
+      dp = d[i + 1] - lam - l * e[i];
       if (fabs(dp) < pfloor) dp = dp < 0.0 ? -pfloor : pfloor;
-      DF[(int64_t)s * ld + r] = 1.0 / dp;
-      for (int i = s; i < t; ++i) {
-        const double l = e[i] / dp;
-        LF[(int64_t)i * ld + r] = l;
-        dp = d[i + 1] - lam - l * e[i];
-        if (fabs(dp) < pfloor) dp = dp < 0.0 ? -pfloor : pfloor;
-        DF[(int64_t)(i + 1) * ld + r] = 1.0 / dp;
+      DF[(int64_t)(i + 1) * ld + j] = 1.0 / dp;
+    }
+    for (int i = s; i <= t; ++i) X[(int64_t)i * ld + j] = start_value(j, i);
+    for (int it = 0; it < 3; ++it) {
+      double xp = X[(int64_t)s * ld + j];
+      double mx = fabs(xp);
+      for (int i = s + 1; i <= t; ++i) {
+        xp = X[(int64_t)i * ld + j] - LF[(int64_t)(i - 1) * ld + j] * xp;
+        X[(int64_t)i * ld + j] = xp;
+        mx = fmax(mx, fabs(xp));
       }
-      for (int i = s; i <= t; ++i) X[(int64_t)i * ld + r] = start_value(j, i);
-      for (int it = 0; it < 3; ++it) {
-        double xp = X[(int64_t)s * ld + r];
-        double mx = fabs(xp);
-        for (int i = s + 1; i <= t; ++i) {
-          xp = X[(int64_t)i * ld + r] - LF[(int64_t)(i - 1) * ld + r] * xp;
-          X[(int64_t)i * ld + r] = xp;
-          mx = fmax(mx, fabs(xp));
+      double sc = mx > 0.0 ? 1.0 / mx : 1.0;
+      mx = 0.0;
+      for (int i = s; i <= t; ++i) {
+        const double v = X[(int64_t)i * ld + j] * sc * DF[(int64_t)i * ld + j];
+        X[(int64_t)i * ld + j] = v;
+        mx = fmax(mx, fabs(v));
+      }
+      sc = mx > 0.0 ? 1.0 / mx : 1.0;
+      xp = X[(int64_t)t * ld + j] * sc;
+      X[(int64_t)t * ld + j] = xp;
+      double ss = xp * xp;
+      for (int i = t - 1; i >= s; --i) {
+        xp = X[(int64_t)i * ld + j] * sc - LF[(int64_t)i * ld + j] * xp;
+        X[(int64_t)i * ld + j] = xp;
+        ss += xp * xp;
+      }
+      const double inv = ss > 0.0 ? 1.0 / sqrt(ss) : 0.0;
+      for (int i = s; i <= t; ++i) X[(int64_t)i * ld + j] *= inv;
+    }
+    bigroot[j] = cl[j];
+  }
+  __threadfence_block();
+  __syncthreads();
+  // the members' rows of Z, written by the whole work-group (coalesced)
+  for (int64_t e0 = threadIdx.x; e0 < (int64_t)kb * n; e0 += blockDim.x) {
+    const int j = (int)(e0 / n), i = (int)(e0 % n);
+    if (bigroot[j] < 0) continue;
+    Zb[(int64_t)j * n + i] =
+        make_double2(i >= bs[j] && i <= bt[j] ? X[(int64_t)i * ld + j] : 0.0, 0.0);
+  }
+}
+
+// ---------------------------------------------------------------- deferred clusters
+constexpr int kBigR = 32;   // rows per step and per target tile
+constexpr int kBigC = 64;   // vector entries per LDS tile
+
+// The deferred-cluster view of matrix b's cluster scratch: block starts
+// bs[kmax] and deferred-cluster roots big[kmax] (the layout of
+// tri_cluster_kernel's ints: bs, bt, cl, nx, roots, mem [kmax], last [n],
+// bigroot [kmax], after three [n][kmax] double blocks).
+__device__ inline void big_view(double* scratch, int64_t sScr, int64_t b, int n, int kmax,
+                                const int*& bs, const int*& big) {
+  const double* base = scratch + b * sScr + 3 * (int64_t)n * kmax;
+  bs = (const int*)base;
+  big = bs + 6 * (int64_t)kmax + n;
+}
+
+// Target rows t0 .. t0 + kBigR - 1 of matrix blockIdx.y (the deferred ones)
+// lose their components along source rows: mode 0, every non-deferred kept
+// row of the same unreduced block; mode 1, the deferred rows of the same
+// block in [row0 - kBigR, row0) (the step just orthonormalised; targets are
+// rows >= row0).  Per chunk of kBigR sources: H = T S^T (thread (a, c),
+// vectors staged through LDS), then T -= H S.  Sources are never targets of
+// the same launch.
+__global__ __launch_bounds__(1024) void big_project_kernel(int n, int kmax, double2* __restrict__ Z,
+                                                           int64_t sZ, const int32_t* __restrict__ kvec,
+                                                           double* __restrict__ scratch, int64_t sScr,
+                                                           int mode, int row0) {
+  __shared__ double Ta[kBigR][kBigC + 1];
+  __shared__ double Sa[kBigR][kBigC + 1];
+  __shared__ double H[kBigR][kBigR + 1];
+  __shared__ int tact[kBigR], tblk[kBigR];
+  __shared__ int any;
+  const int64_t b = blockIdx.y;
+  const int kb = kvec ? min((int)kvec[b], kmax) : kmax;
+  const int *bs, *big;
+  big_view(scratch, sScr, b, n, kmax, bs, big);
+  const int t0 = row0 + (int)blockIdx.x * kBigR;
+  double2* Zb = Z + b * sZ;
+  if (threadIdx.x == 0) any = 0;
+  __syncthreads();
+  if (threadIdx.x < kBigR) {
+    const int r = t0 + (int)threadIdx.x;
+    const int act = r < kb && big[r] >= 0;
+    tact[threadIdx.x] = act;
+    tblk[threadIdx.x] = act ? bs[r] : -1;
+    if (act) any = 1;
+  }
+  __syncthreads();
+  if (!any) return;
+  const int jlo = mode == 0 ? 0 : row0 - kBigR;
+  const int jhi = mode == 0 ? kb : min(row0, kb);
+  const int a = threadIdx.x >> 5, c = threadIdx.x & 31;
+  for (int j0 = jlo; j0 < jhi; j0 += kBigR) {
+    const int j = j0 + c;
+    bool ok = false;
+    if (tact[a] && j < jhi)
+      ok = (mode == 0 ? big[j] < 0 : big[j] >= 0) && bs[j] == tblk[a];
+    __syncthreads();
+    if (threadIdx.x == 0) any = 0;
+    __syncthreads();
+    if (ok) any = 1;
+    __syncthreads();
+    if (!any) continue;
+    double acc = 0.0;
+    for (int i0 = 0; i0 < n; i0 += kBigC) {
+      const int nc = min(kBigC, n - i0);
+      for (int e = threadIdx.x; e < kBigR * kBigC; e += blockDim.x) {
+        const int rr = e / kBigC, ii = e % kBigC;
+        const int tr = t0 + rr, sr = j0 + rr;
+        Ta[rr][ii] = (ii < nc && tact[rr]) ? Zb[(int64_t)tr * n + i0 + ii].x : 0.0;
+        Sa[rr][ii] = (ii < nc && sr < jhi) ? Zb[(int64_t)sr * n + i0 + ii].x : 0.0;
+      }
+      __syncthreads();
+      for (int ii = 0; ii < nc; ++ii) acc += Ta[a][ii] * Sa[c][ii];
+      __syncthreads();
+    }
+    H[a][c] = ok ? acc : 0.0;
+    __syncthreads();
+    for (int i0 = 0; i0 < n; i0 += kBigC) {
+      const int nc = min(kBigC, n - i0);
+      for (int e = threadIdx.x; e < kBigR * kBigC; e += blockDim.x) {
+        const int rr = e / kBigC, ii = e % kBigC;
+        const int sr = j0 + rr;
+        Sa[rr][ii] = (ii < nc && sr < jhi) ? Zb[(int64_t)sr * n + i0 + ii].x : 0.0;
+      }
+      __syncthreads();
+      for (int e = threadIdx.x; e < kBigR * kBigC; e += blockDim.x) {
+        const int rr = e / kBigC, ii = e % kBigC;
+        if (ii < nc && tact[rr]) {
+          double v = 0.0;
+#pragma unroll 8
+          for (int q = 0; q < kBigR; ++q) v += H[rr][q] * Sa[q][ii];
+          Zb[(int64_t)(t0 + rr) * n + i0 + ii].x -= v;
         }
-        double sc = mx > 0.0 ? 1.0 / mx : 1.0;
-        mx = 0.0;
-        for (int i = s; i <= t; ++i) {
-          const double v = X[(int64_t)i * ld + r] * sc * DF[(int64_t)i * ld + r];
-          X[(int64_t)i * ld + r] = v;
-          mx = fmax(mx, fabs(v));
-        }
-        sc = mx > 0.0 ? 1.0 / mx : 1.0;
-        xp = X[(int64_t)t * ld + r] * sc;
-        X[(int64_t)t * ld + r] = xp;
-        double ss = xp * xp;
-        for (int i = t - 1; i >= s; --i) {
-          xp = X[(int64_t)i * ld + r] * sc - LF[(int64_t)i * ld + r] * xp;
-          X[(int64_t)i * ld + r] = xp;
-          ss += xp * xp;
-        }
-        const double inv = ss > 0.0 ? 1.0 / sqrt(ss) : 0.0;
-        for (int i = s; i <= t; ++i) X[(int64_t)i * ld + r] *= inv;
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// The deferred rows among row0 .. row0 + kBigR - 1 of matrix blockIdx.x,
+// orthonormalised per unreduced block: M = Z_s Z_s^T restricted to pairs of
+// one block (identity on the other rows), M = R^T R, Z_s <- R^-T Z_s (one
+// thread per vector entry); modified Gram-Schmidt over the rows, in order,
+// where a pivot falls to 1e-14 of its diagonal entry or below.
+__global__ __launch_bounds__(256) void big_block_kernel(int n, int kmax, double2* __restrict__ Z,
+                                                        int64_t sZ, const int32_t* __restrict__ kvec,
+                                                        double* __restrict__ scratch, int64_t sScr,
+                                                        int row0) {
+  constexpr int kP = kBigR * kBigR / 256;          // entries of M per thread
+  __shared__ double T[kBigR][kBigC + 1];
+  __shared__ double M[kBigR][kBigR + 1];
+  __shared__ double Ri[kBigR][kBigR + 1];
+  __shared__ double dg[kBigR];
+  __shared__ double red[256 / 64];
+  __shared__ int act[kBigR], blk[kBigR];
+  __shared__ int any;
+  const int64_t b = blockIdx.x;
+  const int kb = kvec ? min((int)kvec[b], kmax) : kmax;
+  const int *bs, *big;
+  big_view(scratch, sScr, b, n, kmax, bs, big);
+  double2* Zb = Z + b * sZ;
+  if (threadIdx.x == 0) any = 0;
+  __syncthreads();
+  if (threadIdx.x < kBigR) {
+    const int r = row0 + (int)threadIdx.x;
+    const int ac = r < kb && big[r] >= 0;
+    act[threadIdx.x] = ac;
+    blk[threadIdx.x] = ac ? bs[r] : -1 - (int)threadIdx.x;    // inactive rows pair with none
+    if (ac) any = 1;
+  }
+  __syncthreads();
+  if (!any) return;
+  double acc[kP];
+#pragma unroll
+  for (int u = 0; u < kP; ++u) acc[u] = 0.0;
+  for (int i0 = 0; i0 < n; i0 += kBigC) {
+    const int nc = min(kBigC, n - i0);
+    for (int e = threadIdx.x; e < kBigR * kBigC; e += blockDim.x) {
+      const int rr = e / kBigC, ii = e % kBigC;
+      T[rr][ii] = (ii < nc && act[rr]) ? Zb[(int64_t)(row0 + rr) * n + i0 + ii].x : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < kP; ++u) {
+      const int e = (int)threadIdx.x + u * 256, a = e >> 5, c = e & 31;
+      for (int ii = 0; ii < nc; ++ii) acc[u] += T[a][ii] * T[c][ii];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int u = 0; u < kP; ++u) {
+    const int e = (int)threadIdx.x + u * 256, a = e >> 5, c = e & 31;
+    const bool pair = act[a] && act[c] && blk[a] == blk[c];
+    M[a][c] = pair ? acc[u] : (a == c ? 1.0 : 0.0);
+    if (a == c) dg[a] = M[a][c];
+  }
+  __syncthreads();
+  bool fail = false;
+  for (int k = 0; k < kBigR; ++k) {
+    const double d = M[k][k];
+    if (!(d > 1e-14 * dg[k])) {        // the same decision in every thread
+      fail = true;
+      break;
+    }
+    const double rkk = sqrt(d);
+    __syncthreads();
+    if (threadIdx.x >= (unsigned)k && threadIdx.x < (unsigned)kBigR)
+      M[k][threadIdx.x] = threadIdx.x == (unsigned)k ? rkk : M[k][threadIdx.x] / rkk;
+    __syncthreads();
+    for (int e = threadIdx.x; e < kBigR * kBigR; e += blockDim.x) {
+      const int a = e >> 5, c = e & 31;
+      if (a > k && c >= a) M[a][c] -= M[k][a] * M[k][c];
+    }
+    __syncthreads();
+  }
+  if (!fail) {
+    // Ri = R^-1 (upper triangular; thread j < kBigR builds column j), then
+    // row r of the result is sum_{q <= r} Ri[q][r] z_q
+    if (threadIdx.x < (unsigned)kBigR) {
+      const int j = threadIdx.x;
+      for (int i = kBigR - 1; i >= 0; --i) {
+        double v = i == j ? 1.0 : 0.0;
+        for (int k = i + 1; k <= j; ++k) v -= M[i][k] * Ri[k][j];
+        Ri[i][j] = i <= j ? v / M[i][i] : 0.0;
       }
     }
-    __threadfence_block();
     __syncthreads();
-    // modified Gram-Schmidt over the block, twice
-    for (int pass = 0; pass < 2; ++pass)
-      for (int r = 0; r < m; ++r) {
-        double ss = 0.0;
-        for (int i = s + threadIdx.x; i <= t; i += blockDim.x) {
-          const double v = X[(int64_t)i * ld + r];
-          ss += v * v;
-        }
-        ss = wg_sum(ss, red);
-        const double inv = ss > 0.0 ? 1.0 / sqrt(ss) : 0.0;
-        for (int i = s + threadIdx.x; i <= t; i += blockDim.x) X[(int64_t)i * ld + r] *= inv;
-        __threadfence_block();
-        __syncthreads();
-        for (int c = r + 1 + threadIdx.x; c < m; c += blockDim.x) {
-          double h = 0.0;
-          for (int i = s; i <= t; ++i) h += X[(int64_t)i * ld + r] * X[(int64_t)i * ld + c];
-          for (int i = s; i <= t; ++i) X[(int64_t)i * ld + c] -= h * X[(int64_t)i * ld + r];
-        }
-        __threadfence_block();
-        __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+      double z[kBigR];
+#pragma unroll
+      for (int r = 0; r < kBigR; ++r) z[r] = act[r] ? Zb[(int64_t)(row0 + r) * n + i].x : 0.0;
+#pragma unroll 1
+      for (int r = 0; r < kBigR; ++r) {
+        double v = 0.0;
+#pragma unroll
+        for (int q = 0; q < kBigR; ++q) v += Ri[q][r] * z[q];
+        if (act[r]) Zb[(int64_t)(row0 + r) * n + i].x = v;
       }
-    for (int64_t e0 = threadIdx.x; e0 < (int64_t)m * n; e0 += blockDim.x) {
-      const int r = (int)(e0 / n), i = (int)(e0 % n);
-      Zb[(int64_t)mem[r] * n + i] = make_double2(i >= s && i <= t ? X[(int64_t)i * ld + r] : 0.0,
-                                                 0.0);
     }
+    return;
+  }
+  __syncthreads();
+  // modified Gram-Schmidt over the deferred rows of the step, in order
+  for (int r = 0; r < kBigR; ++r) {
+    if (!act[r]) continue;
+    double2* zr = Zb + (int64_t)(row0 + r) * n;
+    double ss = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) ss += zr[i].x * zr[i].x;
+    ss = wg_sum(ss, red);
+    const double inv = ss > 0.0 ? 1.0 / sqrt(ss) : 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) zr[i].x *= inv;
     __threadfence_block();
     __syncthreads();
+    for (int q = r + 1; q < kBigR; ++q) {
+      if (!act[q] || blk[q] != blk[r]) continue;
+      double2* zq = Zb + (int64_t)(row0 + q) * n;
+      double h = 0.0;
+      for (int i = threadIdx.x; i < n; i += blockDim.x) h += zr[i].x * zq[i].x;
+      h = wg_sum(h, red);
+      for (int i = threadIdx.x; i < n; i += blockDim.x) zq[i].x -= h * zr[i].x;
+      __threadfence_block();
+      __syncthreads();
+    }
   }
 }
 
@@ -638,12 +844,34 @@ extern "C" int aq_tridiag_eigvecs(int64_t n, const double* D, int64_t sD, const 
   const int64_t sScr = cluster_scratch_doubles(n, kc);
   if (kc >= 2) {
     if (n <= 2048)
-      tri_cluster_kernel<2048><<<(unsigned)batch, kTriThreads, 0, stream>>>(
+      tri_cluster_kernel<2048><<<(unsigned)batch, kClThreads, 0, stream>>>(
           (int)n, D, sD, E, sE, W, sW, kvec, (int)kc, (double2*)Z, sZ, scratch, sScr);
     else
-      tri_cluster_kernel<kTriMaxN><<<(unsigned)batch, kTriThreads, 0, stream>>>(
+      tri_cluster_kernel<kTriMaxN><<<(unsigned)batch, kClThreads, 0, stream>>>(
           (int)n, D, sD, E, sE, W, sW, kvec, (int)kc, (double2*)Z, sZ, scratch, sScr);
     AQ_LAUNCH_CHECK();
+    // the clusters' rows, orthonormalised by the passes below (a work-group
+    // with no clustered row in its range exits at once)
+    {
+      const int nstep = (int)cdiv(kc, kBigR);
+      for (int pass = 0; pass < 2; ++pass) {
+        big_project_kernel<<<dim3((unsigned)nstep, (unsigned)batch), 1024, 0, stream>>>(
+            (int)n, (int)kc, (double2*)Z, sZ, kvec, scratch, sScr, 0, 0);
+        AQ_LAUNCH_CHECK();
+        for (int p = 0; p < nstep; ++p) {
+          big_block_kernel<<<(unsigned)batch, 256, 0, stream>>>(
+              (int)n, (int)kc, (double2*)Z, sZ, kvec, scratch, sScr, p * kBigR);
+          AQ_LAUNCH_CHECK();
+          const int row0 = (p + 1) * kBigR;
+          if (row0 < kc) {
+            big_project_kernel<<<dim3((unsigned)cdiv(kc - row0, kBigR), (unsigned)batch), 1024, 0,
+                                 stream>>>((int)n, (int)kc, (double2*)Z, sZ, kvec, scratch, sScr, 1,
+                                           row0);
+            AQ_LAUNCH_CHECK();
+          }
+        }
+      }
+    }
   }
   return AQ_OK;
 }

@@ -116,6 +116,36 @@ def test_tridiag_repeated_eigenvalues_full_rank(n):
     _kron_case([G, G.copy()], [n // 3, n // 2], require=True)
 
 
+@pytest.mark.parametrize("n,k", [(1024, 1024), (2048, 1024)])
+def test_tridiag_large_tail_cluster(n, k):
+    """Spectra over twenty decades (a Haar circuit's growth phase, where the
+    kept rank reaches the Gram order): several hundred eigenvalues chain
+    within the cluster tolerance of ||T|| and form one cluster, which the
+    blocked Gram-Schmidt orthonormalises.  Eigenvalues, residuals, the
+    re-check and the kept subspace as in the other cases; the call's time
+    is printed."""
+    import time
+    rng = np.random.default_rng(n + k)
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+    lam = np.exp(-np.arange(n) * (48.0 / n))
+    G = (Q * lam[None, :]) @ Q.conj().T
+    G = (G + G.conj().T) / 2
+    members = int((lam[:k] < 1e-5).sum())
+    Gb = torch.from_numpy(np.stack([G])).to(DEV).contiguous()
+    tri = GramTridiag(Gb.clone())
+    tri.eigenvalues()
+    tri.vectors([k])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    tri = GramTridiag(Gb.clone())
+    tri.eigenvalues()
+    tri.vectors([k])
+    torch.cuda.synchronize()
+    print(f"  n={n} k={k}: {members} kept eigenvalues below 1e-5 ||G||, "
+          f"solve {1e3 * (time.perf_counter() - t0):.1f} ms", flush=True)
+    _check([G], [k], f"tail cluster n={n}")
+
+
 def _kron_case(Gs, ks, require=False):
     W, Et, err = _solve(Gs, ks)
     for b, (G, k) in enumerate(zip(Gs, ks)):
