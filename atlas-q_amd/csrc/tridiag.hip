@@ -276,9 +276,10 @@ __global__ __launch_bounds__(kTriThreads) void tri_invit_kernel(
 // L D L^T) and runs three solves from a fixed pseudo-random start,
 // normalising its own vector, which is written to Z's row.  The clusters'
 // rows are then orthonormalised by multi-work-group passes over Z
-// (big_project_kernel, big_block_kernel), twice: every clustered row is
-// cleared of the vectors of its unreduced block that are in no cluster,
-// then the clustered rows are orthonormalised in steps of kBigR rows in
+// (big_project_kernel, big_block_kernel), twice: the rows of a cluster of
+// more than kBigR members are cleared of the vectors of their unreduced
+// block that are in no cluster, then the clustered rows are
+// orthonormalised in steps of kBigR rows in
 // eigenvalue order (Cholesky QR of the step's rows, modified Gram-Schmidt
 // where they are too close to dependent for it, then their projection out
 // of every later clustered row of the block, a work-group per kBigR target
@@ -376,6 +377,12 @@ __global__ __launch_bounds__(kClThreads) void tri_cluster_kernel(
       last[bs[j]] = j;
     }
     nroots = nr;
+    // member count per cluster, at its root (mem[root])
+    for (int q = 0; q < nr; ++q) {
+      int c = 0;
+      for (int j = roots[q]; j >= 0; j = nx[j]) ++c;
+      mem[roots[q]] = c;
+    }
   }
   __threadfence_block();
   __syncthreads();
@@ -441,22 +448,25 @@ __global__ __launch_bounds__(kClThreads) void tri_cluster_kernel(
 constexpr int kBigR = 32;   // rows per step and per target tile
 constexpr int kBigC = 64;   // vector entries per LDS tile
 
-// The deferred-cluster view of matrix b's cluster scratch: block starts
-// bs[kmax] and deferred-cluster roots big[kmax] (the layout of
-// tri_cluster_kernel's ints: bs, bt, cl, nx, roots, mem [kmax], last [n],
-// bigroot [kmax], after three [n][kmax] double blocks).
+// The cluster view of matrix b's cluster scratch: block starts bs[kmax],
+// each clustered row's root big[kmax] (-1 for a row in no cluster) and the
+// member count at each root csz[] (the layout of tri_cluster_kernel's ints:
+// bs, bt, cl, nx, roots, mem [kmax], last [n], bigroot [kmax], after three
+// [n][kmax] double blocks).
 __device__ inline void big_view(double* scratch, int64_t sScr, int64_t b, int n, int kmax,
-                                const int*& bs, const int*& big) {
+                                const int*& bs, const int*& big, const int*& csz) {
   const double* base = scratch + b * sScr + 3 * (int64_t)n * kmax;
   bs = (const int*)base;
+  csz = bs + 5 * (int64_t)kmax;           // mem[]: member count at each root
   big = bs + 6 * (int64_t)kmax + n;
 }
 
-// Target rows t0 .. t0 + kBigR - 1 of matrix blockIdx.y (the deferred ones)
-// lose their components along source rows: mode 0, every non-deferred kept
-// row of the same unreduced block; mode 1, the deferred rows of the same
-// block in [row0 - kBigR, row0) (the step just orthonormalised; targets are
-// rows >= row0).  Per chunk of kBigR sources: H = T S^T (thread (a, c),
+// Target rows t0 .. t0 + kBigR - 1 of matrix blockIdx.y (clustered rows)
+// lose their components along source rows: mode 0, the rows of clusters of
+// more than kBigR members lose those along every kept row of the same
+// unreduced block that is in no cluster; mode 1, every clustered row loses
+// those along the clustered rows of the same block in [row0 - kBigR, row0)
+// (the step just orthonormalised; targets are rows >= row0).  Per chunk of kBigR sources: H = T S^T (thread (a, c),
 // vectors staged through LDS), then T -= H S.  Sources are never targets of
 // the same launch.
 __global__ __launch_bounds__(1024) void big_project_kernel(int n, int kmax, double2* __restrict__ Z,
@@ -470,15 +480,17 @@ __global__ __launch_bounds__(1024) void big_project_kernel(int n, int kmax, doub
   __shared__ int any;
   const int64_t b = blockIdx.y;
   const int kb = kvec ? min((int)kvec[b], kmax) : kmax;
-  const int *bs, *big;
-  big_view(scratch, sScr, b, n, kmax, bs, big);
+  const int *bs, *big, *csz;
+  big_view(scratch, sScr, b, n, kmax, bs, big, csz);
   const int t0 = row0 + (int)blockIdx.x * kBigR;
   double2* Zb = Z + b * sZ;
   if (threadIdx.x == 0) any = 0;
   __syncthreads();
   if (threadIdx.x < kBigR) {
     const int r = t0 + (int)threadIdx.x;
-    const int act = r < kb && big[r] >= 0;
+    // mode 0 clears only the rows of clusters longer than a step (a short
+    // cluster's vectors are as accurate as the others', eps ||T|| / gap)
+    const int act = r < kb && big[r] >= 0 && (mode != 0 || csz[big[r]] > kBigR);
     tact[threadIdx.x] = act;
     tblk[threadIdx.x] = act ? bs[r] : -1;
     if (act) any = 1;
@@ -555,8 +567,8 @@ __global__ __launch_bounds__(256) void big_block_kernel(int n, int kmax, double2
   __shared__ int any;
   const int64_t b = blockIdx.x;
   const int kb = kvec ? min((int)kvec[b], kmax) : kmax;
-  const int *bs, *big;
-  big_view(scratch, sScr, b, n, kmax, bs, big);
+  const int *bs, *big, *csz;
+  big_view(scratch, sScr, b, n, kmax, bs, big, csz);
   double2* Zb = Z + b * sZ;
   if (threadIdx.x == 0) any = 0;
   __syncthreads();
