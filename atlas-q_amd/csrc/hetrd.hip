@@ -39,6 +39,11 @@ constexpr int kHvTile = 64;     // hemv tile edge
 constexpr int kStepThreads = 1024;
 
 __device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+__device__ __forceinline__ double2 nt_load(const double2* p) {
+  typedef double dv2 __attribute__((ext_vector_type(2)));
+  const dv2 t = __builtin_nontemporal_load((const dv2*)p);
+  return make_double2(t.x, t.y);
+}
 __device__ __forceinline__ double2 cfma(double2 a, double2 b, double2 c) {
   // c + a b
   return make_double2(c.x + a.x * b.x - a.y * b.y, c.y + a.x * b.y + a.y * b.x);
@@ -130,7 +135,9 @@ __global__ __launch_bounds__(256) void hetrd_hemv_kernel(const double2* __restri
     for (int br = 0; br < 8; ++br) {
       const int r = kHvTile * R + 8 * br + lj;
       const bool ok = crow && r < ii && (!diag || r <= c);
-      a[br] = ok ? rowp[8 * br] : zero;
+      // the triangle is streamed once per column: non-temporal loads keep
+      // it from displacing the partial sums and vectors in the caches
+      a[br] = ok ? nt_load(rowp + 8 * br) : zero;
     }
 #pragma unroll
     for (int br = 0; br < 8; ++br) {
