@@ -243,6 +243,96 @@ __device__ void make_reflector(double2* rowcol, int m, double2* tau_out, double*
   __syncthreads();
 }
 
+// ------------------------------------------------------------ small orders
+// Orders up to kSmallN: the whole reduction by one work-group per matrix
+// (LAPACK's unblocked zhetd2, uplo "upper" on the column-major view, the
+// output format above), with no launch per column.  The leading block is
+// first made fully Hermitian in memory (the elements below the stored
+// triangle get the conjugates of the stored ones), so that per column the
+// matrix-vector product w = tau A v reads coalesced memory rows (four
+// threads per row of A, partial sums in LDS) and the rank-2 update
+// A -= v w^H + w v^H writes both triangles; three passes over the leading
+// block per column, from L2 at these sizes.
+constexpr int kSmallN = 256;
+
+__global__ __launch_bounds__(1024) void hetrd_small_kernel(double2* __restrict__ A, int64_t sA,
+                                                           int n, double* __restrict__ D,
+                                                           double* __restrict__ E,
+                                                           double2* __restrict__ tau) {
+  __shared__ double2 v[kSmallN], w[kSmallN], wp[4][kSmallN];
+  __shared__ double2 scratch[1024 / 64];
+  __shared__ double2 s_tau;
+  __shared__ double s_e;
+  const int64_t b = blockIdx.x;
+  double2* Ab = A + b * sA;
+  // M(r, c) = Ab[c n + r]; the stored triangle is r <= c (memory row c,
+  // elements 0 .. c); fill r > c with conj(M(c, r)) and make the diagonal real
+  for (int e = threadIdx.x; e < n * n; e += blockDim.x) {
+    const int c = e / n, r = e % n;
+    if (r > c) {
+      const double2 x = Ab[(int64_t)r * n + c];
+      Ab[(int64_t)c * n + r] = make_double2(x.x, -x.y);
+    } else if (r == c) {
+      Ab[(int64_t)c * n + c].y = 0.0;
+    }
+  }
+  __threadfence_block();
+  __syncthreads();
+  const int r4 = threadIdx.x & (kSmallN - 1), part = threadIdx.x >> 8;   // 4 x 256 threads
+  for (int k = n - 1; k >= 1; --k) {
+    double2* colk = Ab + (int64_t)k * n;          // M(0 .. n-1, k)
+    make_reflector(colk, k, &s_tau, &s_e, scratch);
+    const double2 tk = s_tau;
+    const double ek = s_e;
+    if (tk.x != 0.0 || tk.y != 0.0) {
+      for (int r = threadIdx.x; r < k; r += blockDim.x) v[r] = colk[r];
+      __syncthreads();
+      // w = tau M v over the leading k x k block (four column ranges)
+      {
+        const int c0 = part * k / 4, c1 = (part + 1) * k / 4;
+        double2 y = make_double2(0.0, 0.0);
+        if (r4 < k)
+          for (int c = c0; c < c1; ++c) y = cfma(Ab[(int64_t)c * n + r4], v[c], y);
+        wp[part][r4] = y;
+      }
+      __syncthreads();
+      double2 dl = make_double2(0.0, 0.0);
+      if (threadIdx.x < k) {
+        const int r = threadIdx.x;
+        double2 y = cadd(cadd(wp[0][r], wp[1][r]), cadd(wp[2][r], wp[3][r]));
+        y = cmul(tk, y);
+        w[r] = y;
+        dl = make_double2(y.x * v[r].x + y.y * v[r].y, y.x * v[r].y - y.y * v[r].x);  // conj(w) v
+      }
+      const double2 dot = block_sum(dl, scratch);     // w^H v (syncs)
+      const double2 a2 = make_double2(-0.5 * (tk.x * dot.x - tk.y * dot.y),
+                                      -0.5 * (tk.x * dot.y + tk.y * dot.x));
+      if (threadIdx.x < k) w[threadIdx.x] = cfma(a2, v[threadIdx.x], w[threadIdx.x]);
+      __syncthreads();
+      // M -= v w^H + w v^H on the leading k x k block
+      for (int e = threadIdx.x; e < k * k; e += blockDim.x) {
+        const int c = e / k, r = e % k;
+        const double2 vr = v[r], wc = w[c], wr = w[r], vc = v[c];
+        double2 m = Ab[(int64_t)c * n + r];
+        m.x -= vr.x * wc.x + vr.y * wc.y + wr.x * vc.x + wr.y * vc.y;
+        m.y -= vr.y * wc.x - vr.x * wc.y + wr.y * vc.x - wr.x * vc.y;
+        Ab[(int64_t)c * n + r] = m;
+      }
+      __threadfence_block();
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      colk[k - 1] = make_double2(ek, 0.0);
+      E[b * (n - 1) + k - 1] = ek;
+      tau[b * (n - 1) + k - 1] = tk;
+      D[b * n + k] = colk[k].x;
+    }
+    __threadfence_block();
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) D[b * n] = Ab[0].x;
+}
+
 // ------------------------------------------------------------ wide column kernels
 // The per-column work between two matrix-vector launches, spread over many
 // work-groups per matrix (each work-group owns 256 rows r):
@@ -580,6 +670,12 @@ extern "C" int aq_hetrd_own(int64_t n, int64_t nb, void* Av, int64_t strideA, do
              "aq_hetrd_own: bad batch or stride");
   AQ_REQUIRE(ws_elems >= aq_hetrd_workspace_elems(n, nb, batch), AQ_ERR_ARG,
              "aq_hetrd_own: workspace too small");
+  if (n <= kSmallN) {
+    hetrd_small_kernel<<<(unsigned)batch, 1024, 0, stream>>>((double2*)Av, strideA, (int)n, D, E,
+                                                             (double2*)tauv);
+    AQ_LAUNCH_CHECK();
+    return AQ_OK;
+  }
   const int64_t nblk_max = cdiv(n, kHvTile);
   const int64_t sW = 3 * nb * n, sPart = 2 * nblk_max * n, sDots = 64 * nblk_max;
   const int64_t nred_max = cdiv(n, kWide), tiles_max = nblk_max * (nblk_max + 1) / 2;
