@@ -249,8 +249,8 @@ __device__ void make_reflector(double2* rowcol, int m, double2* tau_out, double*
 // output format above), with no launch per column.  The leading block is
 // first made fully Hermitian in memory (the elements below the stored
 // triangle get the conjugates of the stored ones), so that per column the
-// matrix-vector product w = tau A v reads coalesced memory rows (four
-// threads per row of A, partial sums in LDS) and the rank-2 update
+// matrix-vector product w = tau A v reads coalesced memory rows (1024 /
+// kSmallN threads per row of A, partial sums in LDS) and the rank-2 update
 // A -= v w^H + w v^H writes both triangles; three passes over the leading
 // block per column, from L2 at these sizes.
 constexpr int kSmallN = 256;
@@ -259,7 +259,8 @@ __global__ __launch_bounds__(1024) void hetrd_small_kernel(double2* __restrict__
                                                            int n, double* __restrict__ D,
                                                            double* __restrict__ E,
                                                            double2* __restrict__ tau) {
-  __shared__ double2 v[kSmallN], w[kSmallN], wp[4][kSmallN];
+  constexpr int kParts = 1024 / kSmallN;
+  __shared__ double2 v[kSmallN], w[kSmallN], wp[kParts][kSmallN];
   __shared__ double2 scratch[1024 / 64];
   __shared__ double2 s_tau;
   __shared__ double s_e;
@@ -278,7 +279,7 @@ __global__ __launch_bounds__(1024) void hetrd_small_kernel(double2* __restrict__
   }
   __threadfence_block();
   __syncthreads();
-  const int r4 = threadIdx.x & (kSmallN - 1), part = threadIdx.x >> 8;   // 4 x 256 threads
+  const int r4 = threadIdx.x % kSmallN, part = threadIdx.x / kSmallN;   // kParts x kSmallN threads
   for (int k = n - 1; k >= 1; --k) {
     double2* colk = Ab + (int64_t)k * n;          // M(0 .. n-1, k)
     make_reflector(colk, k, &s_tau, &s_e, scratch);
@@ -287,9 +288,9 @@ __global__ __launch_bounds__(1024) void hetrd_small_kernel(double2* __restrict__
     if (tk.x != 0.0 || tk.y != 0.0) {
       for (int r = threadIdx.x; r < k; r += blockDim.x) v[r] = colk[r];
       __syncthreads();
-      // w = tau M v over the leading k x k block (four column ranges)
+      // w = tau M v over the leading k x k block (kParts column ranges)
       {
-        const int c0 = part * k / 4, c1 = (part + 1) * k / 4;
+        const int c0 = part * k / kParts, c1 = (part + 1) * k / kParts;
         double2 y = make_double2(0.0, 0.0);
         if (r4 < k)
           for (int c = c0; c < c1; ++c) y = cfma(Ab[(int64_t)c * n + r4], v[c], y);
@@ -299,7 +300,9 @@ __global__ __launch_bounds__(1024) void hetrd_small_kernel(double2* __restrict__
       double2 dl = make_double2(0.0, 0.0);
       if (threadIdx.x < k) {
         const int r = threadIdx.x;
-        double2 y = cadd(cadd(wp[0][r], wp[1][r]), cadd(wp[2][r], wp[3][r]));
+        double2 y = wp[0][r];
+#pragma unroll
+        for (int q = 1; q < kParts; ++q) y = cadd(y, wp[q][r]);
         y = cmul(tk, y);
         w[r] = y;
         dl = make_double2(y.x * v[r].x + y.y * v[r].y, y.x * v[r].y - y.y * v[r].x);  // conj(w) v
