@@ -2,7 +2,9 @@
 // (FP64): the orthonormalisation of the randomized SVD's range-finder
 // sketches.
 //
-//   W = Y^H Y (+ shift I)                 batched MFMA GEMM
+//   W = Y^H Y (+ shift I)                 split-K MFMA GEMM (the reduction
+//                                          over the m rows spread over up
+//                                          to kCqSplits work-group columns)
 //   W = L L^H                              blocked by 64: each diagonal block
 //                                          factored and inverted in LDS by
 //                                          potrf_small_kernel, the panel below
@@ -27,11 +29,25 @@ extern "C" int aq_bgemm(int dtype, int64_t M, int64_t N, int64_t K, double alpha
                         int64_t b_s0, int64_t b_s1, int64_t b_sb, int conj_b, double beta,
                         void* C, int64_t c_s0, int64_t c_s1, int64_t c_sb, int64_t batch,
                         hipStream_t stream);
+extern "C" int aq_cgemm_splitk(int dtype, int prec, int64_t M, int64_t N, int64_t K,
+                               const void* A, int64_t a_s0, int64_t a_s1, int conj_a,
+                               const void* B, int64_t b_s0, int64_t b_s1, int conj_b, void* C,
+                               int64_t c_s0, int64_t c_s1, int64_t splits, void* workspace,
+                               int64_t workspace_bytes, hipStream_t stream);
 
 namespace aq {
 
 constexpr int kCqB = 64;        // block order
 constexpr int kCqThreads = 256;
+constexpr int kCqSplits = 16;   // most K chunks of the Gram matrix
+
+// K chunks of W = Y^H Y: about 128 rows each, at most kCqSplits, and the
+// partial products within 2^24 complex elements
+__host__ inline int64_t cq_splits(int64_t m, int64_t l) {
+  int64_t sp = std::max<int64_t>(1, std::min<int64_t>(kCqSplits, m / 128));
+  while (sp > 1 && sp * l * l > (1LL << 24)) --sp;
+  return sp;
+}
 
 __device__ __forceinline__ double2 zmul(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -81,19 +97,30 @@ __global__ __launch_bounds__(kCqThreads) void potrf_small_kernel(
     }
     __syncthreads();
   }
-  // L^{-1}: column c solves L x = e_c by forward substitution
-  for (int c = tid; c < n; c += kCqThreads) {
-    for (int i = 0; i < c; ++i) x[i][c] = make_double2(0.0, 0.0);
-    x[c][c] = make_double2(1.0 / a[c][c].x, 0.0);
-    for (int i = c + 1; i < n; ++i) {
+  // L^{-1} row by row: x[i][c] = (delta_ic - sum_{c <= p < i} L[i][p] x[p][c]) / L[i][i]
+  // for all c <= i at once, four threads per column (lanes 4c .. 4c + 3)
+  // splitting the sum, combined by lane shuffles
+  for (int e = tid; e < n * n; e += kCqThreads) x[e / n][e % n] = make_double2(0.0, 0.0);
+  __syncthreads();
+  {
+    const int c = tid >> 2, part = tid & 3;   // 64 columns x 4 (kCqThreads = 256)
+    for (int i = 0; i < n; ++i) {
       double2 s = make_double2(0.0, 0.0);
-      for (int p = c; p < i; ++p) {
-        const double2 q = zmul(a[i][p], x[p][c]);
-        s.x += q.x;
-        s.y += q.y;
+      if (c <= i && c < n)
+        for (int p = c + part; p < i; p += 4) {
+          const double2 q = zmul(a[i][p], x[p][c]);
+          s.x += q.x;
+          s.y += q.y;
+        }
+      s.x += __shfl_xor(s.x, 1);
+      s.y += __shfl_xor(s.y, 1);
+      s.x += __shfl_xor(s.x, 2);
+      s.y += __shfl_xor(s.y, 2);
+      if (part == 0 && c <= i && c < n) {
+        const double inv = 1.0 / a[i][i].x;
+        x[i][c] = make_double2(((c == i ? 1.0 : 0.0) - s.x) * inv, -s.y * inv);
       }
-      const double inv = -1.0 / a[i][i].x;
-      x[i][c] = make_double2(s.x * inv, s.y * inv);
+      __syncthreads();
     }
   }
   __syncthreads();
@@ -139,8 +166,9 @@ using namespace aq;
 
 extern "C" int64_t aq_cholqr_workspace_elems(int64_t m, int64_t l) {
   if (m <= 0 || l <= 0) return 0;
-  // W, L, block inverses (l x 64 each: block J at rows 64 J), T (m x 64)
-  return 2 * l * l + l * kCqB + m * kCqB;
+  // W, L, block inverses (l x 64 each: block J at rows 64 J), T (m x 64),
+  // the split-K partial products of W
+  return 2 * l * l + l * kCqB + m * kCqB + cq_splits(m, l) * l * l;
 }
 
 extern "C" int aq_cholqr(int64_t m, int64_t l, const void* Yv, int64_t ldy, void* Qv, int64_t ldq,
@@ -163,8 +191,10 @@ extern "C" int aq_cholqr(int64_t m, int64_t l, const void* Yv, int64_t ldy, void
   double2* T = Li + l * kCqB;   // [m][64]
   AQ_HIP(hipMemsetAsync(info, 0, sizeof(int), stream));
   // W = Y^H Y: A(i, k) = conj(Y[k, i]), B(k, j) = Y[k, j]
-  int st = aq_bgemm(AQ_C128, l, l, m, 1.0, Y, 1, ldy, 0, 1, Y, ldy, 1, 0, 0, 0.0, W, l, 1, 0, 1,
-                    stream);
+  const int64_t sp = cq_splits(m, l);
+  double2* part = T + m * kCqB;
+  int st = aq_cgemm_splitk(AQ_C128, AQ_PREC_NATIVE, l, l, m, Y, 1, ldy, 1, Y, ldy, 1, 0, W, l, 1,
+                           sp, part, sp * l * l * (int64_t)sizeof(double2), stream);
   if (st != AQ_OK) return st;
   if (shift_scale > 0.0) {
     diag_shift_kernel<<<1, kCqThreads, 0, stream>>>(W, l, (int)l, shift_scale);
