@@ -42,6 +42,8 @@ from . import _ffi
 from ._ffi import (COPY_DESC, GEMM_DESC, SAMPLE_DESC, SITE1Q_DESC, RankResult, check, lib)
 from .autotune import Autotuner
 from .eig import JACOBI_MAX, ORTH_TOL, GramTridiag, finite_rows, heevd_batched
+
+BATCHED_JACOBI_MAX = 32
 from .linalg_robust import robust_svd
 from .rank_mlp import heev_small
 
@@ -203,8 +205,16 @@ class BatchedMPS:
 
     def __init__(self, specs: Sequence[Tuple[int, int]], eps_bond: float = 1e-8,
                  device="cuda", autotuner: Autotuner = None, eig_workers: int = 3,
-                 eig_solver: str = "tridiag"):
+                 eig_solver: str = "tridiag", jacobi_max: int = None):
         self.dev = torch.device(device)
+        # Gram orders solved by the Jacobi kernel (each matrix at its own
+        # order); larger ones by the eig_solver path on padded buckets.  At
+        # orders 33-64 the single-work-group reduction, bisection and kept
+        # vectors beat the order-64 Jacobi kernel (population: 205k -> 235k
+        # updates/s; limit 16: 229k)
+        self.jacobi_max = BATCHED_JACOBI_MAX if jacobi_max is None else int(jacobi_max)
+        if not 2 <= self.jacobi_max <= JACOBI_MAX:
+            raise ValueError(f"jacobi_max must be in [2, {JACOBI_MAX}]")
         # host threads (one HIP stream each) for the size buckets above the
         # Jacobi limit: each bucket's solve is a chain of small per-column
         # launches, so buckets solved concurrently overlap their latencies
@@ -396,7 +406,7 @@ class BatchedMPS:
         # the Jacobi buckets (one launch each) run on this one.  The
         # tridiagonal solver gives the eigenvalues first; the vectors of the
         # kept ranks follow the rank rule.
-        big = [sz for sz in buckets if sz > JACOBI_MAX]
+        big = [sz for sz in buckets if sz > self.jacobi_max]
         tri_mode = self.eig_solver == "tridiag"
         if not tri_mode:
             for sz in big:
@@ -416,7 +426,7 @@ class BatchedMPS:
         main = torch.cuda.current_stream(self.dev)
         res = {}
         for sz, (idx, T) in buckets.items():
-            if sz <= JACOBI_MAX:
+            if sz <= self.jacobi_max:
                 W, V = heev_small(T, orders=_h2d(s[idx].astype(np.int64), self.dev))
                 res[sz] = (W, V, finite_rows(W, V))
         res.update(self._on_workers(big, solve_big, main))

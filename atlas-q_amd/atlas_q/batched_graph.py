@@ -177,7 +177,7 @@ class _Step2Q:
             gates["S64"][idx] = _ptrs_rows(S64)
             gates["Sinv"][idx] = _ptrs_rows(Sinv)
             gates["sz"][idx] = sz
-            if sz <= JACOBI_MAX:
+            if sz <= self.bm.jacobi_max:
                 bk["W"] = torch.zeros((B, sz), dtype=torch.float64, device=dev)
                 bk["orders"] = torch.zeros(B, dtype=torch.int64, device=dev)
                 bk["v_off"] = moff
@@ -244,7 +244,7 @@ class _Step2Q:
             sz, B, idx = bk["sz"], bk["B"], bk["idx_np"]
             bk["T"] = mat[bk["t_off"]: bk["t_off"] + B * sz * sz].view(B, sz, sz)
             gates["G"][idx] = _ptrs_rows(bk["T"])
-            if sz <= JACOBI_MAX:
+            if sz <= self.bm.jacobi_max:
                 bk["V"] = mat[bk["v_off"]: bk["v_off"] + B * sz * sz].view(B, sz, sz)
                 gates["V"][idx] = _ptrs_rows(bk["V"])
             else:
@@ -272,7 +272,7 @@ class _Step2Q:
         tris = []
         for bk in self.buckets:
             sz, B, T = bk["sz"], bk["B"], bk["T"]
-            if sz <= JACOBI_MAX:
+            if sz <= self.bm.jacobi_max:
                 torch.index_select(self.order, 0, bk["idx"], out=bk["orders"])
                 check(L.aq_heev_small_orders(sz, _vp(bk["orders"]), _vp(T), sz, sz * sz,
                                              _vp(bk["W"]), _vp(bk["V"]), sz, sz * sz, B, 30,
