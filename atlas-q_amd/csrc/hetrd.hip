@@ -62,8 +62,9 @@ __device__ __forceinline__ double2 cfma_conj(double2 a, double2 b, double2 c) {
 // to the partial buffer part[b][slot][n]: slot C for P1, slot nblk + R for P2.
 // One wave per tile; lane (i, j) = (lane / 8, lane % 8) reads the 8 x 8
 // sub-blocks at rows 8 bc + i, elements 8 br + j (each load instruction is 8
-// full 128-byte rows), accumulates P1 over its rows and P2 over its elements
-// in registers, and the two sums finish with 3 shuffle steps each.
+// full 128-byte rows), accumulates P1 over its rows in registers (finished
+// by 3 shuffle steps at the end) and finishes each memory row's P2 as soon as
+// that row is read (3 shuffle steps, one store).
 __global__ __launch_bounds__(256) void hetrd_hemv_kernel(const double2* __restrict__ A,
                                                          int64_t sA, int n, int ii, int nblk,
                                                          int tiles, double2* __restrict__ part,
@@ -115,16 +116,20 @@ __global__ __launch_bounds__(256) void hetrd_hemv_kernel(const double2* __restri
   const int lane = threadIdx.x & 63, li = lane >> 3, lj = lane & 7;
   const bool diag = (R == C);
   const double2 zero = make_double2(0.0, 0.0);
-  double2 vr[8], acc1[8], acc2[8];
+  double2 vr[8], acc1[8];
   double qs = 0.0;  // this tile's share of v^H A_0 v
 #pragma unroll
   for (int q = 0; q < 8; ++q) {
     const int r = kHvTile * R + 8 * q + lj;
     vr[q] = r < ii ? v[r] : zero;
     acc1[q] = zero;
-    acc2[q] = zero;
   }
-#pragma unroll
+  double2* Pb = part + b * sPart;
+  // two rows of the tile in flight at a time (187 registers: two waves per
+  // SIMD, so one wave's loads overlap another's arithmetic; with all eight
+  // rows' loads hoisted the kernel held 256 registers and ran one wave per
+  // SIMD, 10 % slower)
+#pragma unroll 2
   for (int bc = 0; bc < 8; ++bc) {
     const int c = kHvTile * C + 8 * bc + li;
     const bool crow = c < ii;
@@ -139,6 +144,7 @@ __global__ __launch_bounds__(256) void hetrd_hemv_kernel(const double2* __restri
       // it from displacing the partial sums and vectors in the caches
       a[br] = ok ? nt_load(rowp + 8 * br) : zero;
     }
+    double2 p2 = zero;
 #pragma unroll
     for (int br = 0; br < 8; ++br) {
       const int r = kHvTile * R + 8 * br + lj;
@@ -146,10 +152,16 @@ __global__ __launch_bounds__(256) void hetrd_hemv_kernel(const double2* __restri
       const bool on_diag = diag && r == c;
       if (on_diag) x.y = 0.0;  // the diagonal of a Hermitian matrix is real
       acc1[br] = cfma(x, vc, acc1[br]);
-      if (!on_diag) acc2[bc] = cfma_conj(x, vr[br], acc2[bc]);
+      if (!on_diag) p2 = cfma_conj(x, vr[br], p2);
     }
-    // acc2[bc] is final for this lane: Re(conj(v_c) P2 part) = Re(off-diagonal part)
-    qs += vc.x * acc2[bc].x + vc.y * acc2[bc].y;
+    // p2 is this lane's part of P2[c]: Re(conj(v_c) p2) = Re(off-diagonal part)
+    qs += vc.x * p2.x + vc.y * p2.y;
+    // P2[c] over lj (lanes 8 li .. 8 li + 7), written at once
+    for (int off = 1; off < 8; off <<= 1) {
+      p2.x += __shfl_xor(p2.x, off);
+      p2.y += __shfl_xor(p2.y, off);
+    }
+    if (lj == 0 && crow) Pb[(int64_t)(nblk + R) * n + c] = p2;
   }
   // v^H A_0 v over the tile's stored elements (r <= c): Re(sum conj(v_r) P1_r)
   // holds the diagonal and the real part of the off-diagonal sum once, the
@@ -160,31 +172,19 @@ __global__ __launch_bounds__(256) void hetrd_hemv_kernel(const double2* __restri
     for (int off = 32; off > 0; off >>= 1) qs += __shfl_xor(qs, off);
     if (lane == 0) qpart[b * sQ + t] = qs;
   }
-  // P1 over li (lanes j, j+8, ..), P2 over lj (lanes 8i .. 8i+7)
+  // P1 over li (lanes j, j+8, ..)
 #pragma unroll
   for (int q = 0; q < 8; ++q) {
     for (int off = 8; off < 64; off <<= 1) {
       acc1[q].x += __shfl_xor(acc1[q].x, off);
       acc1[q].y += __shfl_xor(acc1[q].y, off);
     }
-    for (int off = 1; off < 8; off <<= 1) {
-      acc2[q].x += __shfl_xor(acc2[q].x, off);
-      acc2[q].y += __shfl_xor(acc2[q].y, off);
-    }
   }
-  double2* Pb = part + b * sPart;
   if (li == 0) {
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
       const int r = kHvTile * R + 8 * q + lj;
       if (r < ii) Pb[(int64_t)C * n + r] = acc1[q];
-    }
-  }
-  if (lj == 0) {
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const int c = kHvTile * C + 8 * q + li;
-      if (c < ii) Pb[(int64_t)(nblk + R) * n + c] = acc2[q];
     }
   }
 }
@@ -609,7 +609,7 @@ extern "C" int64_t aq_hetrd_workspace_elems(int64_t n, int64_t nb, int64_t batch
   // [V; W; V] | tile partials | dot partials | (norm partials, quadratic-form
   // shares) as doubles
   return batch * (3 * nb * n + 2 * nblk * n + 64 * nblk +
-                  (cdiv(n, 256) + nblk * (nblk + 1) / 2 + 1) / 2 + 1);
+                  (cdiv(n, kWide) + nblk * (nblk + 1) / 2 + 1) / 2 + 1);
 }
 
 // copies the panel's reflector vectors (memory rows p0 .. p0+nb-1, elements
