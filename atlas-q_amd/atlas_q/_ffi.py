@@ -121,6 +121,8 @@ SIGNATURES = {
     "aq_hetrd_workspace_elems": (c_i64, [c_i64, c_i64, c_i64]),
     "aq_hetrd_own": (c_int, [c_i64, c_i64, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_i64, c_i64,
                              c_vp]),
+    "aq_hetrd_own_groups": (c_int, [c_i64, c_i64, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_i64,
+                                    c_i64, c_int, c_vp]),
     "aq_he2hb_panels": (c_i64, [c_i64]),
     "aq_he2hb_workspace_elems": (c_i64, [c_i64, c_i64]),
     "aq_he2hb_own": (c_int, [c_i64, c_vp, c_i64, c_vp, c_vp, c_vp, c_i64, c_i64, c_vp]),

@@ -131,9 +131,12 @@ class GramTridiag:
     tridiagonal form: ``eigenvalues`` gives the spectra by bisection, and
     ``vectors(k)`` the eigenvectors of the k[b] largest eigenvalues of matrix
     b by twisted inverse iteration, back-transformed on MFMA, with the FP64
-    orthogonality re-check of the result."""
+    orthogonality re-check of the result.  ``groups`` (own reduction only):
+    None follows the process-wide stream-group setting; 1 keeps the whole
+    reduction on the current stream, so reductions started from one host
+    thread on different streams overlap."""
 
-    def __init__(self, Gb: torch.Tensor, reduction: str = None):
+    def __init__(self, Gb: torch.Tensor, reduction: str = None, groups: int = None):
         from . import kernels as K
         from ._ffi import check, lib
         if Gb.dtype != torch.complex128 or not Gb.is_contiguous() or Gb.dim() != 3:
@@ -149,9 +152,15 @@ class GramTridiag:
             L = lib()
             nws = int(L.aq_hetrd_workspace_elems(n, HETRD_NB, B))
             ws = torch.empty(nws, dtype=torch.complex128, device=dev)
-            check(L.aq_hetrd_own(n, HETRD_NB, K._ptr(Gb), n * n, K._ptr(self.D), K._ptr(self.E),
-                                 K._ptr(self.tau), K._ptr(ws), nws, B, K._stream(Gb)),
-                  "aq_hetrd_own")
+            if groups is None:
+                check(L.aq_hetrd_own(n, HETRD_NB, K._ptr(Gb), n * n, K._ptr(self.D),
+                                     K._ptr(self.E), K._ptr(self.tau), K._ptr(ws), nws, B,
+                                     K._stream(Gb)), "aq_hetrd_own")
+            else:
+                check(L.aq_hetrd_own_groups(n, HETRD_NB, K._ptr(Gb), n * n, K._ptr(self.D),
+                                            K._ptr(self.E), K._ptr(self.tau), K._ptr(ws), nws,
+                                            B, int(groups), K._stream(Gb)),
+                      "aq_hetrd_own_groups")
             self._ws = ws
         elif reduction == "rocsolver":
             check(lib().aq_hetrd_batched(n, K._ptr(Gb), n * n, K._ptr(self.D), K._ptr(self.E),

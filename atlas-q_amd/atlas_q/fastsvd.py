@@ -218,22 +218,51 @@ def _harvest(mps, feats, k):
         mps.harvest.append((feats.cpu().numpy(), k))
 
 
-def _rank_batched(S32: torch.Tensor, caps, eps: float):
-    """Rank rule (aq_rank_select semantics) on every row of S32 [B, m] in one
-    launch; one device->host copy of the B results."""
-    from ._ffi import RankResult, check, lib
+def _rank_inputs(S32: torch.Tensor, caps):
+    """Device inputs of the batched rank rule over the rows of S32 [B, m]
+    (row pointers, orders, caps) and its output buffer; the uploads
+    synchronise the current stream, so they are made before long work is
+    queued on it."""
+    from ._ffi import RankResult
     import ctypes
     B, m = S32.shape
     dev = S32.device
     ptrs = torch.tensor([S32[b].data_ptr() for b in range(B)], dtype=torch.int64).to(dev)
     mm = torch.full((B,), m, dtype=torch.int64).to(dev)
     cp = torch.tensor([int(c) for c in caps], dtype=torch.int64).to(dev)
+    out = torch.empty(B * ctypes.sizeof(RankResult), dtype=torch.uint8, device=dev)
+    return S32, ptrs, mm, cp, out
+
+
+def _rank_launch(rin, eps: float):
+    """Queue the rank rule (aq_rank_select semantics) on every row and the copy
+    of its results to pinned host memory; returns (host buffer, event)."""
+    from ._ffi import check, lib
+    S32, ptrs, mm, cp, out = rin
+    check(lib().aq_rank_select_batched(0, K._ptr(ptrs), K._ptr(mm), K._ptr(cp), float(eps),
+                                       S32.shape[0], K._ptr(out), K._stream(S32)),
+          "aq_rank_select_batched")
+    host = torch.empty(out.numel(), dtype=torch.uint8, pin_memory=True)
+    host.copy_(out, non_blocking=True)
+    ev = torch.cuda.Event()
+    ev.record()
+    return host, ev
+
+
+def _rank_read(host: torch.Tensor, ev, B: int):
+    from ._ffi import RankResult
+    import ctypes
+    ev.synchronize()
+    raw = host.numpy().tobytes()
     sz = ctypes.sizeof(RankResult)
-    out = torch.empty(B * sz, dtype=torch.uint8, device=dev)
-    check(lib().aq_rank_select_batched(0, K._ptr(ptrs), K._ptr(mm), K._ptr(cp), float(eps), B,
-                                       K._ptr(out), K._stream(S32)), "aq_rank_select_batched")
-    host = out.cpu().numpy().tobytes()
-    return [RankResult.from_buffer_copy(host[b * sz:(b + 1) * sz]) for b in range(B)]
+    return [RankResult.from_buffer_copy(raw[b * sz:(b + 1) * sz]) for b in range(B)]
+
+
+def _rank_batched(S32: torch.Tensor, caps, eps: float):
+    """Rank rule (aq_rank_select semantics) on every row of S32 [B, m] in one
+    launch; one device->host copy of the B results."""
+    host, ev = _rank_launch(_rank_inputs(S32, caps), eps)
+    return _rank_read(host, ev, S32.shape[0])
 
 
 def _gram_group(mps, jobs, st):
@@ -334,6 +363,48 @@ def _gram_group(mps, jobs, st):
 EIG_SOLVER = "tridiag"   # "tridiag": the library's tridiagonal top-k solver; "heevd": rocSOLVER
 
 
+# Shares of a sub-layer's Gram batch factored as separate chunks on their own
+# streams, smallest first, so that a finished chunk's vectors, projection and
+# cores (FP64 MFMA and latency-bound work) could overlap the reductions still
+# running (HBM-bound).  Off by default: measured at chi = 1024 (64 matrices of
+# order 2048) every split was slower than one batch whose reduction runs as
+# two interleaved stream groups (123.8 updates/s; 0.375/0.625: 111.6;
+# 0.25/0.3125/0.4375: 108.9; four chunks: 103.1), with 4, 8 or 16 hardware
+# queues alike: a chunk's reduction on one stream loses the two-group
+# interleave of its per-column launch chain, and the chunks' reductions end
+# together, so little post-processing overlaps them.  Batches below
+# GRAM_CHUNK_MIN are always one chunk.
+GRAM_CHUNKS = ()
+GRAM_CHUNK_MIN = 24
+_CHUNK_STREAMS = {}
+
+
+def _chunk_stream(dev, i: int):
+    """The i-th chunk stream of a device (default priority, created once)."""
+    d = torch.device(dev)
+    key = (d.index if d.index is not None else torch.cuda.current_device(), i)
+    s = _CHUNK_STREAMS.get(key)
+    if s is None:
+        s = _CHUNK_STREAMS[key] = torch.cuda.Stream(device=d)
+    return s
+
+
+def _chunk_sizes(B: int):
+    if B < GRAM_CHUNK_MIN or len(GRAM_CHUNKS) < 2:
+        return [B]
+    sizes = [max(1, int(round(B * f))) for f in GRAM_CHUNKS[:-1]]
+    sizes.append(B - sum(sizes))
+    return [c for c in sizes if c > 0]
+
+
+def _drain(gen):
+    try:
+        while True:
+            next(gen)
+    except StopIteration as stop:
+        return stop.value
+
+
 def _gram_group_tridiag(mps, jobs, st):
     """``_gram_group`` with the eigen-decomposition through the tridiagonal
     form: one batched tridiagonalisation, all eigenvalues by bisection, the
@@ -341,7 +412,52 @@ def _gram_group_tridiag(mps, jobs, st):
     inverse iteration, MFMA back-transformation, FP64 orthogonality
     re-check); a matrix whose vectors fail the re-check takes the per-job
     path.  The projection reads the back-transformed rows, which are the rows
-    of U_k^H."""
+    of U_k^H.  A batch of GRAM_CHUNK_MIN or more is split into chunks
+    (GRAM_CHUNKS) on chunk streams: every chunk's Gram matrices, reduction,
+    bisection and rank rule are queued first, then the chunks are finished in
+    order of size, so the host waits for one chunk's ranks while the others
+    still reduce.  Results are those of one batch: every matrix goes through
+    the same kernels either way."""
+    B = len(jobs)
+    dev = jobs[0].X.device
+    sizes = _chunk_sizes(B)
+    red = RL.Span("eig_zhetrd_batched", "fp64_mfma")
+    if len(sizes) == 1:
+        out = _drain(_tridiag_chunk(mps, jobs, st, red, None))
+        red.close()
+        return out
+    main = torch.cuda.current_stream(dev)
+    parts = []
+    o = 0
+    for c, size in enumerate(sizes):
+        s = _chunk_stream(dev, c)
+        s.wait_stream(main)
+        parts.append((s, o, size, _tridiag_chunk(mps, jobs[o:o + size], st, red, 1)))
+        o += size
+    for s, _, _, gen in parts:
+        with torch.cuda.stream(s):
+            next(gen)
+    out = [None] * B
+    for s, o, size, gen in parts:
+        with torch.cuda.stream(s):
+            out[o:o + size] = _drain(gen)
+    for s, _, _, _ in parts:
+        main.wait_stream(s)
+    for r in out:
+        if r is not None:            # produced on a chunk stream, used on the caller's
+            r[0].record_stream(main)
+            r[1].record_stream(main)
+    red.close()
+    return out
+
+
+def _tridiag_chunk(mps, jobs, st, red, groups):
+    """One chunk of ``_gram_group_tridiag`` on the current stream, as a
+    generator: it queues the Gram matrices, the reduction (timed as a part of
+    ``red``), bisection and the rank rule, yields; reads the ranks, queues the
+    kept vectors and their re-check, yields; reads the re-check and queues the
+    projection and the cores; returns the per-job results (None: per-job
+    path)."""
     from ._ffi import AQ_GROUPED_HERK, COPY_DESC, check, lib
     from .eig import ORTH_TOL, GramTridiag, reduction_bytes
     c128 = torch.complex128
@@ -351,6 +467,8 @@ def _gram_group_tridiag(mps, jobs, st):
     keep = []
     X64 = [j.X.to(c128) for j in jobs]
     Gb = torch.zeros((B, m, m), dtype=c128, device=dev)
+    S32 = torch.empty((B, m), dtype=torch.float32, device=dev)
+    rin = _rank_inputs(S32, [j.cap for j in jobs])
     gd = np.stack([K.gemm_desc(X64[b], X64[b].T, Gb[b], conj_b=True, kind=AQ_GROUPED_HERK)
                    for b in range(B)])
     work = float(sum(4.0 * m * m * x.shape[1] for x in X64))
@@ -359,14 +477,15 @@ def _gram_group_tridiag(mps, jobs, st):
     # algorithmic flops: complex Hermitian tridiagonalisation 16/3 n^3 (and the
     # one-stage method's bytes, which bound it); the Sturm counts of
     # bisection ~53 steps of 4 flops per entry and eigenvalue
-    with RL.timed("eig_zhetrd_batched", B * 16.0 / 3.0 * m ** 3, "fp64_mfma",
-                  nbytes=B * reduction_bytes(m)):
-        tri = GramTridiag(Gb)
+    with red.part(B * 16.0 / 3.0 * m ** 3, nbytes=B * reduction_bytes(m)):
+        tri = GramTridiag(Gb, groups=groups)
     with RL.timed("eig_bisection", B * 53.0 * 4.0 * m * m, "fp64_valu"):
         lam = tri.eigenvalues()
     S64 = torch.sqrt(torch.clamp(lam, min=0.0)).contiguous()          # [B, m] descending
-    ranks = _rank_batched(S64.to(torch.float32).contiguous(), [j.cap for j in jobs],
-                          mps.eps_bond)
+    S32.copy_(S64)
+    rank_host, rank_ev = _rank_launch(rin, mps.eps_bond)
+    yield
+    ranks = _rank_read(rank_host, rank_ev, B)
     out = [None] * B
     good = []
     for b, (j, r) in enumerate(zip(jobs, ranks)):
@@ -385,7 +504,13 @@ def _gram_group_tridiag(mps, jobs, st):
     kmax = max(kv)
     with RL.timed("eig_vectors_backtransform", B * 8.0 * m * m * kmax, "fp64_mfma"):
         Et, err = tri.vectors(kv)
-    errs = err.cpu().tolist()
+    err_host = torch.empty(B, dtype=torch.float64, pin_memory=True)
+    err_host.copy_(err, non_blocking=True)
+    err_ev = torch.cuda.Event()
+    err_ev.record()
+    yield
+    err_ev.synchronize()
+    errs = err_host.tolist()
     st.orth.extend(errs[g[0]] for g in good)
     good = [g for g in good if errs[g[0]] <= ORTH_TOL]
     if not good:
@@ -429,7 +554,10 @@ def _gram_group_tridiag(mps, jobs, st):
     check(lib().aq_grouped_copy(K._ptr(dd), K._ptr(pp), len(cps), total,
                                 int(min(-(-total // 256), 65536)), K._stream(Gb)),
           "aq_grouped_copy")
-    keep += [dd, pp, P, inv, S64, Gb, X64, Et, tri]
+    # the temporaries (held in keep until here) are released in the order of
+    # this chunk's stream, so the caching allocator cannot hand them out before
+    # the launches above ran
+    keep += [dd, pp, P, inv, S64, Gb, X64, Et, tri, rin]
     for b, r in res:
         out[b] = r
     return out

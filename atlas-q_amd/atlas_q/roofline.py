@@ -48,6 +48,46 @@ def timed(name: str, work: float, bound: str, nbytes: float = 0.0):
     _records.append((name, s, e, float(work), bound, float(nbytes)))
 
 
+class Span:
+    """One logical call made of parts on several streams (for instance the
+    same step of several batches that overlap): ``part()`` times each part on
+    the current stream, and the record's duration runs from the earliest part
+    start to the latest part end."""
+
+    def __init__(self, name: str, bound: str):
+        self.name, self.bound = name, bound
+        self.starts, self.ends = [], []
+        self.work = self.nbytes = 0.0
+
+    @contextlib.contextmanager
+    def part(self, work: float, nbytes: float = 0.0):
+        if not _enabled:
+            yield
+            return
+        s = torch.cuda.Event(enable_timing=True)
+        e = torch.cuda.Event(enable_timing=True)
+        s.record()
+        yield
+        e.record()
+        self.starts.append(s)
+        self.ends.append(e)
+        self.work += float(work)
+        self.nbytes += float(nbytes)
+
+    def close(self) -> None:
+        if _enabled and self.starts:
+            _records.append((self.name, self.starts, self.ends, self.work, self.bound,
+                             self.nbytes))
+
+
+def _elapsed_ms(s, e) -> float:
+    """Event-pair duration; for a span, earliest start to latest end."""
+    if isinstance(s, list):
+        ref = s[0]
+        return (max(ref.elapsed_time(x) for x in e) - min(ref.elapsed_time(x) for x in s))
+    return s.elapsed_time(e)
+
+
 def summary(window_s: float = None, units: int = None):
     """Per-kernel figures of the recorded launches.  The roofline object names
     the kernel (or library call) with the largest total time, whoever wrote
@@ -63,7 +103,7 @@ def summary(window_s: float = None, units: int = None):
     for name, s, e, work, bound, nb in _records:
         a = agg[name]
         a[0] += 1
-        a[1] += s.elapsed_time(e)
+        a[1] += _elapsed_ms(s, e)
         a[2] += work
         a[3] = bound
         a[4] += nb

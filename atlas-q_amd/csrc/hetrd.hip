@@ -663,10 +663,12 @@ struct HetrdGroup {
 };
 }  // namespace aq
 
-extern "C" int aq_hetrd_own(int64_t n, int64_t nb, void* Av, int64_t strideA, double* D,
-                            double* E, void* tauv, void* ws, int64_t ws_elems, int64_t batch,
-                            hipStream_t stream) {
+extern "C" int aq_hetrd_own_groups(int64_t n, int64_t nb, void* Av, int64_t strideA,
+                                   double* D, double* E, void* tauv, void* ws, int64_t ws_elems,
+                                   int64_t batch, int groups, hipStream_t stream) {
   AQ_REQUIRE(Av && D && E && tauv && ws, AQ_ERR_ARG, "aq_hetrd_own: null pointer");
+  AQ_REQUIRE(groups >= 1 && groups <= kHetrdStreams, AQ_ERR_ARG,
+             "aq_hetrd_own: groups %d out of range", groups);
   AQ_REQUIRE(n >= 2 && n <= 8192 && nb >= 1 && nb <= 32, AQ_ERR_ARG,
              "aq_hetrd_own: n %lld / nb %lld out of range", (long long)n, (long long)nb);
   AQ_REQUIRE(batch >= 1 && batch <= 65535 && strideA >= n * n, AQ_ERR_ARG,
@@ -692,7 +694,7 @@ extern "C" int aq_hetrd_own(int64_t n, int64_t nb, void* Av, int64_t strideA, do
   // streaming matrix-vector launch.
   // (from two matrices on: at small batches every launch is latency-bound,
   // and the second group's chain fills the first one's gaps)
-  const int ngroups = batch >= 2 ? g_hetrd_max_groups : 1;
+  const int ngroups = batch >= 2 ? groups : 1;
   hipStream_t ws_streams[kHetrdStreams];
   HetrdGroup g[kHetrdStreams];
   if (ngroups > 1) {
@@ -793,6 +795,13 @@ extern "C" int aq_hetrd_own(int64_t n, int64_t nb, void* Av, int64_t strideA, do
     AQ_HIP(hipEventDestroy(ready));
   }
   return AQ_OK;
+}
+
+extern "C" int aq_hetrd_own(int64_t n, int64_t nb, void* Av, int64_t strideA, double* D,
+                            double* E, void* tauv, void* ws, int64_t ws_elems, int64_t batch,
+                            hipStream_t stream) {
+  return aq_hetrd_own_groups(n, nb, Av, strideA, D, E, tauv, ws, ws_elems, batch,
+                             g_hetrd_max_groups, stream);
 }
 
 // Stream groups of the following aq_hetrd_own calls (process-wide): 1 keeps

@@ -133,6 +133,9 @@ def parse():
                          "timed fast-path region (dispatch markers for trace filtering)")
     ap.add_argument("--sampler-only", action="store_true",
                     help="only run the sampler measurement and print its JSON object")
+    ap.add_argument("--gram-chunks", default=None, metavar="F,F,...",
+                    help="shares of a sub-layer's Gram batch factored as overlapped chunks "
+                         "(default: the library's setting; 1 = one batch)")
     ap.add_argument("--seed", type=int, default=2025)
     ap.add_argument("--gemm-probe", action="store_true",
                     help="only run 5 FP64 Gram GEMMs X X^H of the bench's 2048 x 2048 theta "
@@ -1015,6 +1018,9 @@ def main():
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
+    from atlas_q import fastsvd
+    if a.gram_chunks:
+        fastsvd.GRAM_CHUNKS = tuple(float(f) for f in a.gram_chunks.split(","))
     if a.sampler_only:
         print(json.dumps({"sampler": sampler_bench(torch, a, dev)}), flush=True)
         return
@@ -1072,7 +1078,8 @@ def main():
                                      "achieved": e["achieved"], "unit": e["unit"],
                                      "frac": e["frac"]}
         eig["ms_per_update"] = sum(v["total_ms"] for v in eig["calls"].values()) / a.steps
-        eig["solver"] = __import__("atlas_q.fastsvd", fromlist=["x"]).EIG_SOLVER
+        eig["solver"] = fastsvd.EIG_SOLVER
+        eig["gram_chunks"] = list(fastsvd.GRAM_CHUNKS)
         orth = getattr(getattr(fast, "factor_stats", None), "orth", [])
         eig["max_orthogonality_error"] = max(orth) if orth else None
     ks = fast.statistics.logs["k_star"][a.warmup:]

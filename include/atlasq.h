@@ -320,6 +320,13 @@ int aq_orth_band_reduce(const void* Q, int64_t ntile, const int32_t* kvec, int64
 int64_t aq_hetrd_workspace_elems(int64_t n, int64_t nb, int64_t batch);
 int aq_hetrd_own(int64_t n, int64_t nb, void* A, int64_t strideA, double* D, double* E,
                  void* tau, void* ws, int64_t ws_elems, int64_t batch, hipStream_t stream);
+/* aq_hetrd_own with the group count given per call instead of the
+ * process-wide setting: groups = 1 runs the whole reduction on `stream` (no
+ * worker streams, so several calls from one host thread on different
+ * streams proceed independently and overlap), 2 as aq_hetrd_own. */
+int aq_hetrd_own_groups(int64_t n, int64_t nb, void* A, int64_t strideA, double* D, double* E,
+                        void* tau, void* ws, int64_t ws_elems, int64_t batch, int groups,
+                        hipStream_t stream);
 /* First stage of the two-stage tridiagonalisation (the library's own; no
  * reference counterpart, as for the reduction above): Hermitian A [batch]
  * [n][n] (row-major, lower triangle read; overwritten) = Q1 B Q1^H with B of
