@@ -43,6 +43,7 @@ COPY_DESC = _np.dtype([("src", "u8"), ("dst", "u8"), ("row_scale", "u8"), ("col_
                        ("rows", "i8"), ("cols", "i8"), ("s0", "i8"), ("s1", "i8"),
                        ("conj", "i4"), ("pad", "i4")])
 AQ_GROUPED_GEMM, AQ_GROUPED_THETA, AQ_GROUPED_HERK = 0, 1, 2
+AQ_GROUPED_LAYOUT_ROWS32 = 16   # aq_grouped_gemm layout bit: 32-row tiles of 128 threads
 LAYER_GATE = _np.dtype([("siteA", "u8"), ("siteB", "u8"), ("X", "u8"), ("X64", "u8"), ("G", "u8"),
                         ("V", "u8"), ("S64", "u8"), ("Sinv", "u8"), ("P", "u8"), ("dims", "u8"),
                         ("bond", "i8"), ("sz", "i8"), ("left", "i8"), ("vfmt", "i8"),

@@ -1,13 +1,15 @@
 """Launch-shape autotuner for the grouped persistent kernels.
 
 The grouped kernels take the number of persistent work-groups as a free
-parameter.  For each (kernel family, work-size bucket) the tuner times the
-candidate grids on the real work list with HIP events (idempotent launches
-only), keeps the fastest, and persists the table as JSON
-(``$ATLASQ_AUTOTUNE_CACHE``, default ``autotune_cache.json`` next to this
-file).  Untuned keys use a heuristic: enough work-groups for every tile up to
-four per CU (256 CUs) for GEMM families, and up to 16 waves per CU for
-element-wise families.
+parameter, and the grouped GEMM also its tile shape (64-row tiles of 256
+threads or 32-row tiles of 128 threads; a group of few-row problems wastes
+half as much of each 32-row tile).  For each (kernel family, work-size
+bucket) the tuner times the candidate shapes on the real work list with HIP
+events (idempotent launches only), keeps the fastest, and persists the table
+as JSON (``$ATLASQ_AUTOTUNE_CACHE``, default ``autotune_cache.json`` next to
+this file).  Untuned keys use a heuristic: 64-row tiles, enough work-groups
+for every tile up to four per CU (256 CUs) for GEMM families, and up to 16
+waves per CU for element-wise families.
 """
 from __future__ import annotations
 
@@ -79,6 +81,35 @@ class Autotuner:
             if ms < best_ms:
                 best, best_ms = g, ms
         self.table[k] = best
+        return best
+
+    def shape(self, key: str, totals: dict, trial_fn=None):
+        """(tile rows, grid) for a grouped GEMM whose tile count is totals[rows]
+        for each candidate tile height; trial_fn(rows, grid) launches it."""
+        base = totals[64]
+        k = f"{key}:shape:{_bucket(base)}"
+        if k in self.table:
+            rows, g = self.table[k]
+            return int(rows), int(g)
+        if not (self.tune and trial_fn is not None):
+            return 64, self.heuristic(key, base)
+        best, best_ms = None, float("inf")
+        for rows, total in sorted(totals.items(), reverse=True):
+            for g in CANDIDATES["gemm"]:
+                g = max(1, min(g, total))
+                trial_fn(rows, g)  # warm
+                s = torch.cuda.Event(enable_timing=True)
+                e = torch.cuda.Event(enable_timing=True)
+                s.record()
+                for _ in range(3):
+                    trial_fn(rows, g)
+                e.record()
+                e.synchronize()
+                ms = s.elapsed_time(e) / 3
+                self.trials.append((k, (rows, g), ms))
+                if ms < best_ms:
+                    best, best_ms = (rows, g), ms
+        self.table[k] = list(best)
         return best
 
     def save(self, path: Path = None):

@@ -316,20 +316,24 @@ class BatchedMPS:
             return None
         paired = descs["kind"] == _ffi.AQ_GROUPED_THETA
         tn = np.where(paired, _cdiv(descs["N"] // 2, 64), _cdiv(descs["N"], 128))
-        tiles = _cdiv(descs["M"], 64) * tn
-        pre, total = _prefix(tiles)
-        dd, pd = _upload(descs, self.dev), _h2d(pre, self.dev)
-        return dtype_code, dd, pd, len(descs), total, layout, key
+        dd = _upload(descs, self.dev)
+        shapes = {}
+        for rows in (64, 32):              # the grouped GEMM's two tile heights
+            pre, total = _prefix(_cdiv(descs["M"], rows) * tn)
+            shapes[rows] = (_h2d(pre, self.dev), total)
+        return dtype_code, dd, shapes, len(descs), layout, key
 
     def _launch_grouped(self, prep):
         if prep is None:
             return
-        dtype_code, dd, pd, count, total, layout, key = prep
+        dtype_code, dd, shapes, count, layout, key = prep
 
-        def run(g):
-            check(lib().aq_grouped_gemm(dtype_code, _vp(dd), _vp(pd), count, total, layout,
+        def run(rows, g):
+            pd, total = shapes[rows]
+            lay = layout | (_ffi.AQ_GROUPED_LAYOUT_ROWS32 if rows == 32 else 0)
+            check(lib().aq_grouped_gemm(dtype_code, _vp(dd), _vp(pd), count, total, lay,
                                         g, _stream()), "aq_grouped_gemm")
-        run(self.tuner.grid(key, total, run))
+        run(*self.tuner.shape(key, {r: t for r, (_, t) in shapes.items()}, run))
 
     def apply_2q_layer(self, ops: Sequence[Sequence[Tuple[int, np.ndarray]]]):
         """ops[j] = [(bond, U4), ...] with disjoint bonds within each MPS."""

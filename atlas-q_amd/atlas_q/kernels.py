@@ -379,10 +379,12 @@ def gemm_desc(A: torch.Tensor, B: torch.Tensor, C: torch.Tensor, conj_a=False, c
 
 
 def grouped_gemm_descs(descs: np.ndarray, dtype: torch.dtype, device, keep=None,
-                       xcd_ranges: bool = False) -> None:
+                       xcd_ranges: bool = False, rows32: bool = False) -> None:
     """Launch prepared descriptors (one dtype, one operand layout) in one
     grouped launch; the uploads are appended to ``keep`` (the caller holds
-    them until the stream has run) or synchronised here."""
+    them until the stream has run) or synchronised here.  ``rows32``: 32-row
+    tiles of 128 threads (an AQ_GROUPED_HERK problem is then a full
+    product)."""
     if len(descs) == 0:
         return
     ka = bool(np.all(descs["a_s1"] == 1))
@@ -390,8 +392,13 @@ def grouped_gemm_descs(descs: np.ndarray, dtype: torch.dtype, device, keep=None,
     if not ka and np.any(descs["a_s1"] == 1) or not nb and np.any(descs["b_s1"] == 1):
         raise ValueError("grouped_gemm: operands must share their layouts")
     herk = descs["kind"] == _ffi.AQ_GROUPED_HERK
-    tiles = np.where(herk, herk_tiles(descs["M"]),
-                     (-(-descs["M"] // 64)) * (-(-descs["N"] // 128)))
+    if rows32:
+        paired = descs["kind"] == _ffi.AQ_GROUPED_THETA
+        tn = np.where(paired, -(-(descs["N"] // 2) // 64), -(-descs["N"] // 128))
+        tiles = (-(-descs["M"] // 32)) * tn
+    else:
+        tiles = np.where(herk, herk_tiles(descs["M"]),
+                         (-(-descs["M"] // 64)) * (-(-descs["N"] // 128)))
     pre = np.zeros(len(tiles), np.int64)
     pre[1:] = np.cumsum(tiles[:-1])
     total = int(tiles.sum())
@@ -405,7 +412,8 @@ def grouped_gemm_descs(descs: np.ndarray, dtype: torch.dtype, device, keep=None,
     check(lib().aq_grouped_gemm(_code(dtype), ctypes.c_void_p(dd.data_ptr()),
                                 ctypes.c_void_p(pd.data_ptr()), len(descs), total,
                                 int(ka) | (int(nb) << 1) | (int(no_theta) << 2)
-                                | (int(bool(xcd_ranges)) << 3), grid,
+                                | (int(bool(xcd_ranges)) << 3)
+                                | (_ffi.AQ_GROUPED_LAYOUT_ROWS32 if rows32 else 0), grid,
                                 ctypes.c_void_p(st.cuda_stream)), "aq_grouped_gemm")
     if keep is None:
         st.synchronize()

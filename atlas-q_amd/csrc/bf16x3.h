@@ -294,4 +294,108 @@ __device__ __forceinline__ void cgemm_mainloop_planes(const PlanesA& pa, const P
   }
 }
 
+
+// Taller variant: work-group of 128 WR threads (WR waves of 32 rows x two
+// column waves), tile 32 WR rows x two halves of 64 columns; the next k step's
+// 16-byte chunks are prefetched into registers while this one is computed.
+// (Two register sets, loads issued two steps ahead, measured no faster at
+// WR = 4: 248 VGPRs, same time.)
+template <int WR>
+struct alignas(16) SmemPlanes {
+  uint16_t a[4][32 * WR][kBKb + kPadB];
+  uint16_t b[4][kBN][kBKb + kPadB];
+};
+
+template <int WR>
+__device__ __forceinline__ void cgemm_mainloop_planes_w(const PlanesA& pa, const PlanesB& pb,
+                                                        int64_t Kp, int64_t m0, int64_t j0,
+                                                        SmemPlanes<WR>& sm, Frag<float>& fr) {
+  constexpr int BK = kBKb;
+  constexpr int NT = 128 * WR;
+  constexpr int ROWS = 32 * WR;
+  constexpr int AP = 4 * ROWS * 4 / NT;        // A chunks per thread (4)
+  constexpr int BP = 4 * 2 * 64 * 4 / NT;      // B chunks per thread (16 / WR)
+  static_assert(AP * NT == 4 * ROWS * 4 && BP * NT == 2048, "chunk split");
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = wave % WR, wn = wave / WR;
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      fr.re[0][0][h][r] = 0;
+      fr.im[0][0][h][r] = 0;
+    }
+  gptr_u32x4 ga[AP];
+  gptr_u32x4 gb[BP];
+  uint16_t* la[AP];
+  uint16_t* lb[BP];
+#pragma unroll
+  for (int i = 0; i < AP; ++i) {
+    const int c = tid + NT * i;
+    const int q = c / (ROWS * 4), row = (c >> 2) % ROWS, kq = c & 3;
+    ga[i] = (gptr_u32x4)(pa.p + q * pa.plane + (m0 + row) * pa.Kp + 8 * kq);
+    la[i] = &sm.a[q][row][8 * kq];
+  }
+#pragma unroll
+  for (int i = 0; i < BP; ++i) {
+    const int c = tid + NT * i;
+    const int q = c >> 9, h = (c >> 8) & 1, col = (c >> 2) & 63, kq = c & 3;
+    gb[i] = (gptr_u32x4)(pb.p + q * pb.plane + h * pb.half + (j0 + col) * pb.Kp + 8 * kq);
+    lb[i] = &sm.b[q][h * kBJ + col][8 * kq];
+  }
+  u32x4 ra[AP], rb[BP];
+  const int64_t steps = Kp / BK;
+  auto fetch = [&](int64_t s) {
+#pragma unroll
+    for (int i = 0; i < AP; ++i) ra[i] = ga[i][s * (BK / 8)];
+#pragma unroll
+    for (int i = 0; i < BP; ++i) rb[i] = gb[i][s * (BK / 8)];
+  };
+  fetch(0);
+  for (int64_t s = 0; s < steps; ++s) {
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < AP; ++i) *reinterpret_cast<u32x4*>(la[i]) = ra[i];
+#pragma unroll
+    for (int i = 0; i < BP; ++i) *reinterpret_cast<u32x4*>(lb[i]) = rb[i];
+    __syncthreads();
+    if (s + 1 < steps) fetch(s + 1);
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 16) {
+      const int kof = kk + 8 * (lane >> 5);
+      const int row = 32 * wm + (lane & 31);
+      const bf16x8 arh = load8(&sm.a[0][row][kof]);
+      const bf16x8 arl = load8(&sm.a[1][row][kof]);
+      const bf16x8 aih = load8(&sm.a[2][row][kof]);
+      const bf16x8 ail = load8(&sm.a[3][row][kof]);
+      const bf16x8 naih = neg8(aih), nail = neg8(ail);
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int col = h * kBJ + 32 * wn + (lane & 31);
+        const bf16x8 brh = load8(&sm.b[0][col][kof]);
+        const bf16x8 brl = load8(&sm.b[1][col][kof]);
+        const bf16x8 bih = load8(&sm.b[2][col][kof]);
+        const bf16x8 bil = load8(&sm.b[3][col][kof]);
+        f32x16 cr = fr.re[0][0][h], ci = fr.im[0][0][h];
+        cr = mma_bf16(arh, brl, cr);
+        cr = mma_bf16(arl, brh, cr);
+        cr = mma_bf16(naih, bil, cr);
+        cr = mma_bf16(nail, bih, cr);
+        cr = mma_bf16(arh, brh, cr);
+        cr = mma_bf16(naih, bih, cr);
+        ci = mma_bf16(arh, bil, ci);
+        ci = mma_bf16(arl, bih, ci);
+        ci = mma_bf16(aih, brl, ci);
+        ci = mma_bf16(ail, brh, ci);
+        ci = mma_bf16(arh, bih, ci);
+        ci = mma_bf16(aih, brh, ci);
+        fr.re[0][0][h] = cr;
+        fr.im[0][0][h] = ci;
+      }
+    }
+  }
+}
+
 }  // namespace aq

@@ -214,10 +214,12 @@ template <typename T> struct Frag {
 
 // One LDS tile, two barriers per k step (a two-slot ring with one barrier
 // measured slower: the barrier is not what bounds the loop).
-template <typename T> struct alignas(16) Smem {
+// BM: rows of the work-group tile (64, or 32 for the half-height tile of the
+// grouped kernel's tile-shape option).
+template <typename T, int BM = kBM> struct alignas(16) Smem {
   static constexpr int BK = TileK<T>::value;
   static constexpr int PAD = 4;
-  T a[2][BK][kBM + PAD];
+  T a[2][BK][BM + PAD];
   T b[2][BK][kBN + PAD];
 };
 
@@ -225,19 +227,24 @@ template <typename T> struct alignas(16) Smem {
 // n_fast_b: the B functor is contiguous along n (else along k).
 // K is the reduction length the functors were built with: element k is
 // in range when k < K (rows / columns are checked by the functors).
-template <typename T, bool KFastA, bool NFastB, bool ThreeM = false, class AOp, class BOp>
+// WR: waves per tile column (2: the 64-row tile of 256 threads; 1: a 32-row
+// tile of 128 threads, two waves side by side).
+template <typename T, bool KFastA, bool NFastB, bool ThreeM = false, int WR = 2, class AOp,
+          class BOp>
 __device__ __forceinline__ void cgemm_mainloop(const AOp& aop, const BOp& bop, const ColMap& cm,
-                                               int64_t K, int64_t m0, int64_t j0, Smem<T>& sm,
-                                               Frag<T>& fr) {
+                                               int64_t K, int64_t m0, int64_t j0,
+                                               Smem<T, 32 * WR>& sm, Frag<T>& fr) {
   using M = Mfma<T>;
-  constexpr int BK = Smem<T>::BK;
+  constexpr int BK = Smem<T, 32 * WR>::BK;
   constexpr int E = M::E;
-  constexpr int A_PER = kBM * BK / kThreads;
-  constexpr int B_PER = kBN * BK / kThreads;
+  constexpr int BM = 32 * WR;
+  constexpr int NT = 128 * WR;
+  constexpr int A_PER = BM * BK / NT;
+  constexpr int B_PER = kBN * BK / NT;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
-  const int wm = wave & 1, wn = wave >> 1;
+  const int wm = wave % WR, wn = wave / WR;
 
 #pragma unroll
   for (int rb = 0; rb < Frag<T>::RB; ++rb)
@@ -264,7 +271,7 @@ __device__ __forceinline__ void cgemm_mainloop(const AOp& aop, const BOp& bop, c
   int bkk[B_PER], btc[B_PER];
 #pragma unroll
   for (int i = 0; i < B_PER; ++i) {
-    int e = tid + i * kThreads;
+    int e = tid + i * NT;
     int tc, kk;
     if (NFastB) { tc = e % kBN; kk = e / kBN; }
     else        { kk = e % BK;  tc = e / BK; }
@@ -277,9 +284,9 @@ __device__ __forceinline__ void cgemm_mainloop(const AOp& aop, const BOp& bop, c
   int amm[A_PER], akk[A_PER];
 #pragma unroll
   for (int i = 0; i < A_PER; ++i) {
-    int e = tid + i * kThreads;
+    int e = tid + i * NT;
     if (KFastA) { akk[i] = e % BK; amm[i] = e / BK; }
-    else        { amm[i] = e % kBM; akk[i] = e / kBM; }
+    else        { amm[i] = e % BM; akk[i] = e / BM; }
     aptr[i] = aop.base_of(m0 + amm[i], aok[i]);
   }
   const int64_t a_ks = aop.kstep(), b_ks = bop.kstep();
@@ -387,14 +394,14 @@ __device__ __forceinline__ void cgemm_mainloop(const AOp& aop, const BOp& bop, c
 }
 
 // Visit every accumulator element of this thread: f(tile_row, h, tile_jj, re, im)
-// where tile_row in [0, 64), tile_jj in [0, 64).
-template <typename T, class F>
+// where tile_row in [0, 32 WR), tile_jj in [0, 64).
+template <typename T, int WR = 2, class F>
 __device__ __forceinline__ void frag_foreach(const Frag<T>& fr, F&& f) {
   using M = Mfma<T>;
   constexpr int E = M::E;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const int wm = wave & 1, wn = wave >> 1;
+  const int wm = wave % WR, wn = wave / WR;
 #pragma unroll
   for (int rb = 0; rb < Frag<T>::RB; ++rb)
 #pragma unroll

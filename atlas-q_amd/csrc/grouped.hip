@@ -28,12 +28,17 @@ __device__ __forceinline__ int find_owner(const int64_t* __restrict__ prefix, in
 // WithTheta = false: the group holds no AQ_GROUPED_THETA problem and the gate
 // epilogue is not compiled in (it doubles the kernel's register footprint
 // and halves its occupancy).
-template <typename T, bool KFastA, bool NFastB, bool WithTheta>
-__global__ __launch_bounds__(kThreads, 2) void grouped_gemm_kernel(const aq_gemm_desc_t* __restrict__ d,
+// WR: tile height in waves (2: 64-row tiles of 256 threads; 1: 32-row tiles of
+// 128 threads, which waste half as much of a tile on problems of few rows).
+// The host's tile counts follow the tile height; with WR = 1 an
+// AQ_GROUPED_HERK problem is computed as a full product (every tile).
+template <typename T, bool KFastA, bool NFastB, bool WithTheta, int WR>
+__global__ __launch_bounds__(128 * WR, 2) void grouped_gemm_kernel(const aq_gemm_desc_t* __restrict__ d,
                                                                 const int64_t* __restrict__ prefix,
                                                                 int P, int64_t total,
                                                                 int xcd_ranges) {
-  __shared__ Smem<T> sm;
+  constexpr int BM = 32 * WR;
+  __shared__ Smem<T, BM> sm;
   // Work-groups are dispatched to the 8 XCDs round-robin (XCD = blockIdx.x
   // mod 8) and every XCD has its own L2.  With xcd_ranges (and a grid that is
   // a multiple of 8) each XCD takes a contiguous range of tiles, so the tiles
@@ -54,7 +59,7 @@ __global__ __launch_bounds__(kThreads, 2) void grouped_gemm_kernel(const aq_gemm
     const int64_t half = paired ? g.N / 2 : g.N;
     const int64_t tn = paired ? cdiv(half, kBJ) : cdiv(g.N, kBN);
     int64_t m0, j0;
-    if (g.kind == AQ_GROUPED_HERK) {
+    if (WR == 2 && g.kind == AQ_GROUPED_HERK) {
       // tiles meeting the lower triangle of a square C: tile row bm holds
       // floor(bm/2) + 1 tiles, so rows (2q, 2q+1) hold 2(q+1) tiles and the
       // tiles before them number q(q+1)
@@ -65,7 +70,7 @@ __global__ __launch_bounds__(kThreads, 2) void grouped_gemm_kernel(const aq_gemm
       m0 = (2 * q + r / (q + 1)) * kBM;
       j0 = (r % (q + 1)) * kBJ;
     } else {
-      m0 = (local / tn) * kBM;
+      m0 = (local / tn) * BM;
       j0 = (local % tn) * kBJ;
     }
     // a tile past the problem (device-planned problems are smaller than the
@@ -76,7 +81,7 @@ __global__ __launch_bounds__(kThreads, 2) void grouped_gemm_kernel(const aq_gemm
     ColMap cm{paired, half};
     Frag<T> fr;
     __syncthreads();  // LDS reuse across tiles
-    cgemm_mainloop<T, KFastA, NFastB>(aop, bop, cm, g.K, m0, j0, sm, fr);
+    cgemm_mainloop<T, KFastA, NFastB, false, WR>(aop, bop, cm, g.K, m0, j0, sm, fr);
     c2_t<T>* C = (c2_t<T>*)g.C;
     if constexpr (WithTheta) {
     if (paired) {
@@ -84,7 +89,7 @@ __global__ __launch_bounds__(kThreads, 2) void grouped_gemm_kernel(const aq_gemm
       using M = Mfma<T>;
       constexpr int E = M::E;
       const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-      const int wm = wave & 1, wn = wave >> 1;
+      const int wm = wave % WR, wn = wave / WR;
       T ur[16], ui[16];
 #pragma unroll
       for (int i = 0; i < 16; ++i) {
@@ -128,7 +133,7 @@ __global__ __launch_bounds__(kThreads, 2) void grouped_gemm_kernel(const aq_gemm
       continue;
     }
     }
-    frag_foreach<T>(fr, [&](int row, int h, int jj, T re, T im) {
+    frag_foreach<T, WR>(fr, [&](int row, int h, int jj, T re, T im) {
       const int64_t m = m0 + row;
       const int64_t n = 2 * j0 + h * kBJ + jj;
       if (m < g.M && n < g.N) {
@@ -243,19 +248,23 @@ extern "C" int aq_grouped_gemm(int dtype, const aq_gemm_desc_t* descs, const int
   if (rc) return rc;
   AQ_REQUIRE(total_tiles > 0, AQ_ERR_ARG, "aq_grouped_gemm: no tiles");
   AQ_REQUIRE(grid >= 1 && grid <= 65536, AQ_ERR_ARG, "aq_grouped_gemm: grid out of range");
-  AQ_REQUIRE(layout >= 0 && layout <= 15, AQ_ERR_ARG, "aq_grouped_gemm: layout 0..15");
+  AQ_REQUIRE(layout >= 0 && layout <= 31, AQ_ERR_ARG, "aq_grouped_gemm: layout 0..31");
   const bool ka = layout & 1, nb = layout & 2, no_theta = layout & 4;
   const int xcd = (layout & 8) ? 1 : 0;
+  const bool half = layout & 16;  // 32-row tiles
+#define AQ_GG1(T, KA, NB, TH, WR)                                                              \
+  grouped_gemm_kernel<T, KA, NB, TH, WR><<<grid, 128 * WR, 0, stream>>>(descs, prefix,         \
+                                                                        num_problems,          \
+                                                                        total_tiles, xcd)
 #define AQ_GG(T, KA, NB)                                                                       \
   do {                                                                                         \
-    if (no_theta)                                                                              \
-      grouped_gemm_kernel<T, KA, NB, false><<<grid, kThreads, 0, stream>>>(descs, prefix,      \
-                                                                           num_problems,       \
-                                                                           total_tiles, xcd);  \
-    else                                                                                       \
-      grouped_gemm_kernel<T, KA, NB, true><<<grid, kThreads, 0, stream>>>(descs, prefix,       \
-                                                                          num_problems,        \
-                                                                          total_tiles, xcd);   \
+    if (half) {                                                                                \
+      if (no_theta) AQ_GG1(T, KA, NB, false, 1);                                               \
+      else AQ_GG1(T, KA, NB, true, 1);                                                         \
+    } else {                                                                                   \
+      if (no_theta) AQ_GG1(T, KA, NB, false, 2);                                               \
+      else AQ_GG1(T, KA, NB, true, 2);                                                         \
+    }                                                                                          \
   } while (0)
   if (dtype == AQ_C64) {
     if (ka && nb) AQ_GG(float, true, true);
@@ -270,6 +279,7 @@ extern "C" int aq_grouped_gemm(int dtype, const aq_gemm_desc_t* descs, const int
     else AQ_GG(double, false, false);
   }
 #undef AQ_GG
+#undef AQ_GG1
   AQ_LAUNCH_CHECK();
   return AQ_OK;
 }

@@ -21,7 +21,9 @@ template <typename T> struct Gate4 {
   T im[16];
 };
 
-template <typename T, class FR>
+// WR: waves per column of the work-group (wave w owns rows 32 (w % WR) and
+// columns 32 (w / WR) of both halves).
+template <typename T, class FR, int WR = 2>
 __device__ __forceinline__ void theta_epilogue(const FR& fr, const Gate4<T>& g, c2_t<T>* X,
                                                int64_t chiL, int64_t chiR, int64_t m0,
                                                int64_t j0) {
@@ -29,7 +31,7 @@ __device__ __forceinline__ void theta_epilogue(const FR& fr, const Gate4<T>& g, 
   constexpr int E = M::E;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const int wm = wave & 1, wn = wave >> 1;
+  const int wm = wave % WR, wn = wave / WR;
   const int64_t ld = 2 * chiR;
 #pragma unroll
   for (int rb = 0; rb < FR::RB; ++rb)
@@ -98,7 +100,9 @@ __global__ __launch_bounds__(kThreads) void theta_bf16x3_kernel(
   theta_epilogue<float>(fr, g, X, chiL, chiR, m0, j0);
 }
 
-// A(m, k) = A[m lda + k] (k contiguous) -> planes; one thread per 8 k values.
+// A(m, k) = A[m lda + k] (k contiguous) -> planes; one thread per 8 k values
+// (16-byte loads of two complex values where the row's 8 values are in range
+// and 16-byte aligned, i.e. always for an even lda and aligned A).
 __global__ __launch_bounds__(256) void split_rows_kernel(const float2* __restrict__ A, int64_t M,
                                                          int64_t K, int64_t lda, int conj,
                                                          uint16_t* __restrict__ P, int64_t Mp,
@@ -109,11 +113,22 @@ __global__ __launch_bounds__(256) void split_rows_kernel(const float2* __restric
        idx += (int64_t)gridDim.x * blockDim.x) {
     const int64_t m = idx / kq, k0 = (idx % kq) * 8;
     float2 x[8];
+    const float2* src = A + m * lda + k0;
+    if (m < M && k0 + 8 <= K && ((uintptr_t)src & 15) == 0) {
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      const bool ok = m < M && k0 + e < K;
-      x[e] = ok ? A[m * lda + k0 + e] : make_float2(0.f, 0.f);
-      if (conj) x[e].y = -x[e].y;
+      for (int e = 0; e < 4; ++e) {
+        const float4 v = reinterpret_cast<const float4*>(src)[e];
+        x[2 * e] = make_float2(v.x, v.y);
+        x[2 * e + 1] = make_float2(v.z, v.w);
+      }
+    } else {
+#pragma unroll
+      for (int e = 0; e < 8; ++e)
+        x[e] = (m < M && k0 + e < K) ? src[e] : make_float2(0.f, 0.f);
+    }
+    if (conj) {
+#pragma unroll
+      for (int e = 0; e < 8; ++e) x[e].y = -x[e].y;
     }
     uint4 o[4];
     split8(x, o);
@@ -174,6 +189,36 @@ __global__ __launch_bounds__(kThreads, 2) void theta_planes_kernel(PlanesA pa, P
   Frag<float> fr;
   cgemm_mainloop_planes(pa, pb, Kp, m0, j0, sm, fr);
   theta_epilogue<float>(fr, g, X, chiL, chiR, m0, j0);
+}
+
+// The same on a tile of 32 WR rows with 128 WR threads, on a 1-D grid of
+// gx x gy tiles.  Work-groups go to the 8 XCDs round-robin (XCD = id mod 8)
+// and each XCD has its own L2; with gx % 4 == 0 and gy % 2 == 0 the XCD
+// blocks tile the grid as 4 x 2 blocks of (gx/4) x (gy/2) tiles, so an XCD
+// reads the A strips of gx/4 row tiles and the B strips of gy/2 column tiles
+// (instead of every B strip: a third less operand traffic at 16 x 16 tiles).
+template <int WR>
+__global__ __launch_bounds__(128 * WR, 2) void theta_planes_w_kernel(PlanesA pa, PlanesB pb,
+                                                                   int64_t Kp, Gate4<float> g,
+                                                                   float2* __restrict__ X,
+                                                                   int64_t chiL, int64_t chiR,
+                                                                   int gx, int gy) {
+  __shared__ SmemPlanes<WR> sm;
+  const int id = blockIdx.x;
+  int bx, by;
+  if (gx % 4 == 0 && gy % 2 == 0) {
+    const int xcd = id % 8, local = id / 8, bw = gx / 4;
+    bx = (xcd % 4) * bw + local % bw;
+    by = (xcd / 4) * (gy / 2) + local / bw;
+  } else {
+    bx = id % gx;
+    by = id / gx;
+  }
+  const int64_t m0 = (int64_t)bx * (32 * WR);
+  const int64_t j0 = (int64_t)by * kBJ;
+  Frag<float> fr;
+  cgemm_mainloop_planes_w<WR>(pa, pb, Kp, m0, j0, sm, fr);
+  theta_epilogue<float, Frag<float>, WR>(fr, g, X, chiL, chiR, m0, j0);
 }
 
 template <typename T, bool KFastA, bool NFastB>
@@ -401,7 +446,7 @@ struct ThetaPlanes {
 };
 static ThetaPlanes theta_planes(int64_t chiL, int64_t chiM, int64_t chiR) {
   ThetaPlanes t;
-  t.Mp = cdiv(2 * chiL, kBM) * kBM;
+  t.Mp = cdiv(2 * chiL, 2 * kBM) * 2 * kBM;   // whole tiles of the 128-row variants
   t.Jp = cdiv(chiR, kBJ) * kBJ;
   t.Kp = cdiv(chiM, kBKb) * kBKb;
   t.a_elems = 4 * t.Mp * t.Kp;
@@ -447,9 +492,20 @@ extern "C" int aq_theta_planes(const void* A, const void* B, const double* gate_
   }
   PlanesA pa{Pa, t.Kp, t.Mp * t.Kp};
   PlanesB pb{Pb, t.Kp, 2 * t.Jp * t.Kp, t.Jp * t.Kp};
-  dim3 grid((unsigned)(t.Mp / kBM), (unsigned)(t.Jp / kBJ));
-  theta_planes_kernel<<<grid, kThreads, 0, stream>>>(pa, pb, t.Kp, make_gate<float>(gate_host),
-                                                      (float2*)X, chiL, chiR);
+  const Gate4<float> gg = make_gate<float>(gate_host);
+  // 128-row tiles (eight waves, half the operand traffic per MFMA of the
+  // 64-row tile) where they still give every CU a work-group; 64-row tiles
+  // below that (theta of 2048 x 2048: 81 vs 91 us alone on the GPU)
+  const int64_t wide_tiles = (t.Mp / 128) * (t.Jp / kBJ);
+  if (wide_tiles >= 256) {
+    const int gx = (int)(t.Mp / 128), gy = (int)(t.Jp / kBJ);
+    theta_planes_w_kernel<4><<<(unsigned)(gx * gy), 512, 0, stream>>>(pa, pb, t.Kp, gg,
+                                                                      (float2*)X, chiL, chiR,
+                                                                      gx, gy);
+  } else {
+    dim3 grid((unsigned)cdiv(2 * chiL, kBM), (unsigned)(t.Jp / kBJ));
+    theta_planes_kernel<<<grid, kThreads, 0, stream>>>(pa, pb, t.Kp, gg, (float2*)X, chiL, chiR);
+  }
   AQ_LAUNCH_CHECK();
   return AQ_OK;
 }

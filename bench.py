@@ -133,6 +133,8 @@ def parse():
                          "timed fast-path region (dispatch markers for trace filtering)")
     ap.add_argument("--sampler-only", action="store_true",
                     help="only run the sampler measurement and print its JSON object")
+    ap.add_argument("--theta-probe", type=int, default=0, metavar="N",
+                    help="N BF16x3 theta calls at bond --chi (random operands), for profiling")
     ap.add_argument("--gram-chunks", default=None, metavar="F,F,...",
                     help="shares of a sub-layer's Gram batch factored as overlapped chunks "
                          "(default: the library's setting; 1 = one batch)")
@@ -752,6 +754,8 @@ def batched_bench(torch, a, dev):
                          f"({c_up} two-site updates), one gate at a time, torch CPU with {cores} "
                          f"threads; a per-MPS rate, not extrapolated to the population"}
     chis = [int(bm.dims[j].max()) for j in range(len(specs))]
+    from atlas_q.autotune import Autotuner
+    shapes = {k: v for k, v in sorted(Autotuner.default().table.items()) if ":shape:" in k}
     out = {"workload": f"{len(specs)} MPS, qubits log-uniform 12-96, caps log-uniform 16-256, "
                        f"RY/RZ + CZ-ladder ansatz x {a.batched_layers} layers, complex64, "
                        f"cutoff 1e-8, {S} shots each",
@@ -760,6 +764,12 @@ def batched_bench(torch, a, dev):
            "max_bond_reached": max(chis),
            "eig_fallbacks": int(bm.eig_fallbacks),
            "eig_solver_above_order_64": a.batched_eig,
+           "grouped_gemm_shapes": {
+               "what": "autotuned (tile rows, work-groups) per grouped GEMM family and "
+                       "log2 work-size bucket",
+               "chosen": shapes,
+               "rows32_share": (sum(1 for v in shapes.values() if v[0] == 32) / len(shapes)
+                                if shapes else None)},
            "graph_replay": graph,
            "shots": S * len(specs), "sample_s": t_samp,
            "shots_per_s": S * len(specs) / t_samp, "sampler": "grouped sweep, HIP-graph replay",
@@ -1006,6 +1016,22 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"hetrd_probe_batch": a.hetrd_probe, "order": n,
                           "ms_per_call": 1e3 * (time.perf_counter() - t0)}), flush=True)
+        return
+    if a.theta_probe:
+        g = torch.Generator(device="cuda").manual_seed(a.seed)
+        A = torch.randn((a.chi, 2, a.chi), dtype=torch.complex64, device="cuda", generator=g)
+        B = torch.randn((a.chi, 2, a.chi), dtype=torch.complex64, device="cuda", generator=g)
+        gate = np.random.default_rng(a.seed).standard_normal(32)
+        X = K.theta(A, B, gate, prec="bf16x3")
+        torch.cuda.synchronize()
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record()
+        for _ in range(a.theta_probe):
+            K.theta(A, B, gate, prec="bf16x3", out=X)
+        e.record()
+        e.synchronize()
+        print(json.dumps({"theta_probe_calls": a.theta_probe, "chi": a.chi,
+                          "us_per_call": s.elapsed_time(e) / a.theta_probe * 1e3}), flush=True)
         return
     if a.copy_probe:
         src = torch.ones(a.copy_probe << 18, dtype=torch.float32, device="cuda")

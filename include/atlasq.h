@@ -424,7 +424,10 @@ typedef struct {
  * plain GEMM); bit3 = XCD-contiguous tile ranges: with a grid that is a
  * multiple of 8 each of the 8 XCDs (round-robin dispatch) takes a contiguous
  * range of tiles (fewer L2 misses; measured slightly slower for the FP64
- * Gram / projection groups, so off by default). */
+ * Gram / projection groups, so off by default); bit4 = 32-row tiles of 128
+ * threads instead of 64-row tiles of 256 (the tile counts above with
+ * ceil(M/32) rows of tiles; an AQ_GROUPED_HERK problem is then computed as
+ * a full product, ceil(M/32) * ceil(N/128) tiles). */
 int aq_grouped_gemm(int dtype, const aq_gemm_desc_t* descs, const int64_t* prefix,
                     int num_problems, int64_t total_tiles, int layout, int grid,
                     hipStream_t stream);

@@ -209,12 +209,13 @@ def test_apply_1q_transposed_convention(dtype):
     assert _relerr(out, ref) < TOL[dtype]
 
 
+@pytest.mark.parametrize("rows32", [False, True])
 @pytest.mark.parametrize("xcd", [False, True])
 @pytest.mark.parametrize("M,N", [(128, 256), (100, 73), (200, 200), (130, 520)])
-def test_grouped_herk_lower_triangle(M, N, xcd):
+def test_grouped_herk_lower_triangle(M, N, xcd, rows32):
     """AQ_GROUPED_HERK computes every element with row >= col of X X^H (the
     triangle the eigensolver reads); problems of different orders in one
-    launch."""
+    launch; with 32-row tiles as a full product."""
     from atlas_q._ffi import AQ_GROUPED_HERK
     torch.manual_seed(M + N)
     Xs = [torch.randn(M, N, dtype=torch.complex128, device=DEV),
@@ -222,20 +223,22 @@ def test_grouped_herk_lower_triangle(M, N, xcd):
     Gs = [torch.zeros(x.shape[0], x.shape[0], dtype=torch.complex128, device=DEV) for x in Xs]
     descs = np.stack([K.gemm_desc(x, x.T, g, conj_b=True, kind=AQ_GROUPED_HERK)
                       for x, g in zip(Xs, Gs)])
-    K.grouped_gemm_descs(descs, torch.complex128, DEV, xcd_ranges=xcd)
+    K.grouped_gemm_descs(descs, torch.complex128, DEV, xcd_ranges=xcd, rows32=rows32)
     for x, g in zip(Xs, Gs):
         ref = x @ x.conj().T
         low = torch.tril(torch.ones_like(ref.real, dtype=torch.bool))
         assert torch.allclose(g[low], ref[low], atol=1e-10 * N)
 
 
+@pytest.mark.parametrize("rows32", [False, True])
 @pytest.mark.parametrize("dtype", [torch.complex64, torch.complex128])
 @pytest.mark.parametrize("b_layout", ["row", "col"])
-def test_grouped_gemm_mixed_problems(dtype, b_layout):
+def test_grouped_gemm_mixed_problems(dtype, b_layout, rows32):
     """One grouped launch over problems of unrelated shapes (ragged M, N, K,
     K below one k tile, K = 1), per-problem conjugation flags, and A read with
     a negative row stride (rows reversed, as the projection reads the
-    eigensolver's output): every element matches the float64 product."""
+    eigensolver's output): every element matches the float64 product, with
+    either tile height."""
     shapes = [(1, 1, 1), (67, 3, 130), (130, 33, 257), (64, 16, 128), (5, 200, 9)]
     flags = [(False, False), (True, False), (False, True), (True, True), (True, False)]
     g = torch.Generator().manual_seed(11 + (b_layout == "col"))
@@ -256,7 +259,7 @@ def test_grouped_gemm_mixed_problems(dtype, b_layout):
         refs.append((A64.conj() if ca else A64) @ (B64.conj() if cb else B64))
         outs.append(C)
         keep += [A, Af, B]
-    K.grouped_gemm_descs(np.stack(descs), dtype, DEV)
+    K.grouped_gemm_descs(np.stack(descs), dtype, DEV, rows32=rows32)
     for C, ref in zip(outs, refs):
         assert _relerr(C, ref) < 10 * TOL[dtype]
 
