@@ -107,7 +107,7 @@ def eigh_desc(G: torch.Tensor):
 
 
 # ---------------------------------------------------------------- tridiagonal path
-NB_BACK = 64       # Householder vectors per block of the MFMA back-transformation
+NB_BACK = 128      # Householder vectors per block of the MFMA back-transformation
 HETRD_NB = 32      # panel width of the library's tridiagonal reduction
 REDUCTION = "own"  # "own": the library's kernels; "rocsolver": rocSOLVER's zhetrd
 ORTH_TILE = 64     # rows per diagonal tile of the orthogonality re-check

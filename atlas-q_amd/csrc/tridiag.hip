@@ -701,20 +701,24 @@ __global__ __launch_bounds__(256) void hetrd_vblk_kernel(int n, int nb, int nblk
 // Forward compact-WY factor of P^H = H_{i0}^H ... H_{i0+nb-1}^H from the
 // reflectors' Gram matrix Gv[a, c] = v_a^H v_c (nb x nb, row-major):
 // T'[j, j] = conj(tau_j), T'[0:j, j] = -conj(tau_j) T'[0:j, 0:j] Gv[0:j, j].
-// One work-group of nb threads per (matrix, block); Tf row-major nb x nb.
+// One work-group of nb threads per (matrix, block, sub-block): the blocks are
+// ld x ld (Gv, Tf), block p's reflectors start at p * ld, and sub-block z
+// covers its reflectors z nb .. z nb + nb - 1 (the diagonal nb x nb block of
+// Gv and Tf at (z nb, z nb)).
 __global__ void hetrd_tfactor_kernel(int n, int nb, int nblk, const double2* __restrict__ tau,
                                      int64_t stau, const double2* __restrict__ Gv,
-                                     double2* __restrict__ Tf) {
+                                     double2* __restrict__ Tf, int ld) {
   __shared__ double2 t[64][65];
   const int64_t b = blockIdx.y;
   const int p = blockIdx.x;
+  const int off = (int)blockIdx.z * nb;
   const int a = threadIdx.x;
-  const int64_t off = (b * nblk + p) * (int64_t)nb * nb;
-  const double2* G = Gv + off;
+  const int64_t base = (b * nblk + p) * (int64_t)ld * ld + (int64_t)off * ld + off;
+  const double2* G = Gv + base;
   for (int c = 0; c < nb; ++c) t[a][c] = make_double2(0.0, 0.0);
   __syncthreads();
   for (int j = 0; j < nb; ++j) {
-    const int64_t i = (int64_t)p * nb + j;
+    const int64_t i = (int64_t)p * ld + off + j;
     double2 tj = make_double2(0.0, 0.0);
     if (i <= n - 2) {
       tj = tau[b * stau + i];
@@ -723,7 +727,7 @@ __global__ void hetrd_tfactor_kernel(int n, int nb, int nblk, const double2* __r
     // row a < j of column j: -conj(tau_j) sum_{c < j} T'[a, c] Gv[c, j]
     if (a < j) {
       double2 acc = make_double2(0.0, 0.0);
-      for (int c = a; c < j; ++c) acc = cadd(acc, cmul(t[a][c], G[(int64_t)c * nb + j]));
+      for (int c = a; c < j; ++c) acc = cadd(acc, cmul(t[a][c], G[(int64_t)c * ld + j]));
       const double2 v = cmul(tj, acc);
       t[a][j] = make_double2(-v.x, -v.y);
     } else if (a == j) {
@@ -731,7 +735,39 @@ __global__ void hetrd_tfactor_kernel(int n, int nb, int nblk, const double2* __r
     }
     __syncthreads();
   }
-  for (int c = 0; c < nb; ++c) Tf[off + (int64_t)a * nb + c] = t[a][c];
+  for (int c = 0; c < nb; ++c) Tf[base + (int64_t)a * ld + c] = t[a][c];
+}
+
+// Joins the two diagonal factors of a 2h x 2h block (h <= 64) into the
+// block's factor: T'_12 = -T'_11 Gv_12 T'_22, T'_21 = 0 (the column recursion
+// above, split at h).  One work-group of 256 threads per (matrix, block).
+__global__ __launch_bounds__(256) void hetrd_tmerge_kernel(int h, int nblk,
+                                                           const double2* __restrict__ Gv,
+                                                           double2* __restrict__ Tf) {
+  __shared__ double2 m[64][65];
+  const int64_t b = blockIdx.y;
+  const int p = blockIdx.x;
+  const int ld = 2 * h;
+  const int64_t base = (b * nblk + p) * (int64_t)ld * ld;
+  const double2* G = Gv + base;
+  double2* T = Tf + base;
+  // m = Gv_12 T'_22 (T'_22 upper triangular)
+  for (int e = threadIdx.x; e < h * h; e += blockDim.x) {
+    const int a = e / h, c = e % h;
+    double2 acc = make_double2(0.0, 0.0);
+    for (int l = 0; l <= c; ++l)
+      acc = cadd(acc, cmul(G[(int64_t)a * ld + h + l], T[(int64_t)(h + l) * ld + h + c]));
+    m[a][c] = acc;
+  }
+  __syncthreads();
+  // T'_12 = -T'_11 m (T'_11 upper triangular); T'_21 = 0
+  for (int e = threadIdx.x; e < h * h; e += blockDim.x) {
+    const int a = e / h, c = e % h;
+    double2 acc = make_double2(0.0, 0.0);
+    for (int l = a; l < h; ++l) acc = cadd(acc, cmul(T[(int64_t)a * ld + l], m[l][c]));
+    T[(int64_t)a * ld + h + c] = make_double2(-acc.x, -acc.y);
+    T[(int64_t)(h + a) * ld + c] = make_double2(0.0, 0.0);
+  }
 }
 
 // err[b] = max over a, c < k_b of |Q[b, tile, a, c] - delta_ac| where the
@@ -891,7 +927,7 @@ extern "C" int aq_tridiag_eigvecs(int64_t n, const double* D, int64_t sD, const 
 extern "C" int aq_hetrd_vblocks(int64_t n, int64_t nb, const void* A, int64_t strideA,
                                 void* V, int64_t batch, hipStream_t stream) {
   AQ_REQUIRE(A && V, AQ_ERR_ARG, "aq_hetrd_vblocks: null pointer");
-  AQ_REQUIRE(n >= 2 && nb >= 1 && nb <= 64 && batch >= 1 && batch <= 65535, AQ_ERR_ARG,
+  AQ_REQUIRE(n >= 2 && nb >= 1 && nb <= 128 && batch >= 1 && batch <= 65535, AQ_ERR_ARG,
              "aq_hetrd_vblocks: bad sizes");
   const int64_t nblk = cdiv(n - 1, nb);
   dim3 grid((unsigned)std::min<int64_t>(cdiv(nb * n, 256), 64), (unsigned)nblk,
@@ -905,12 +941,29 @@ extern "C" int aq_hetrd_vblocks(int64_t n, int64_t nb, const void* A, int64_t st
 extern "C" int aq_hetrd_tfactors(int64_t n, int64_t nb, const void* tau, int64_t stau,
                                  const void* Gv, void* Tf, int64_t batch, hipStream_t stream) {
   AQ_REQUIRE(tau && Gv && Tf, AQ_ERR_ARG, "aq_hetrd_tfactors: null pointer");
-  AQ_REQUIRE(n >= 2 && nb >= 1 && nb <= 64 && batch >= 1 && batch <= 65535, AQ_ERR_ARG,
-             "aq_hetrd_tfactors: bad sizes");
+  AQ_REQUIRE(n >= 2 && nb >= 1 && (nb <= 64 || (nb <= 128 && nb % 2 == 0)) && batch >= 1 &&
+                 batch <= 65535,
+             AQ_ERR_ARG, "aq_hetrd_tfactors: bad sizes");
   const int64_t nblk = cdiv(n - 1, nb);
-  dim3 grid((unsigned)nblk, (unsigned)batch);
-  hetrd_tfactor_kernel<<<grid, (unsigned)nb, 0, stream>>>(
-      (int)n, (int)nb, (int)nblk, (const double2*)tau, stau, (const double2*)Gv, (double2*)Tf);
+  if (nb <= 64) {
+    dim3 grid((unsigned)nblk, (unsigned)batch, 1);
+    hetrd_tfactor_kernel<<<grid, (unsigned)nb, 0, stream>>>((int)n, (int)nb, (int)nblk,
+                                                            (const double2*)tau, stau,
+                                                            (const double2*)Gv, (double2*)Tf,
+                                                            (int)nb);
+    AQ_LAUNCH_CHECK();
+    return AQ_OK;
+  }
+  // blocks of 65..128 reflectors: the two halves' factors, then their join
+  const int h = (int)(nb / 2);
+  dim3 grid((unsigned)nblk, (unsigned)batch, 2);
+  hetrd_tfactor_kernel<<<grid, (unsigned)h, 0, stream>>>((int)n, h, (int)nblk,
+                                                         (const double2*)tau, stau,
+                                                         (const double2*)Gv, (double2*)Tf,
+                                                         (int)nb);
+  AQ_LAUNCH_CHECK();
+  dim3 gm((unsigned)nblk, (unsigned)batch);
+  hetrd_tmerge_kernel<<<gm, 256, 0, stream>>>(h, (int)nblk, (const double2*)Gv, (double2*)Tf);
   AQ_LAUNCH_CHECK();
   return AQ_OK;
 }

@@ -278,7 +278,8 @@ int aq_bgemm_lower(int64_t M, int64_t K, double alpha, const void* A, int64_t a_
  * earlier ones (LAPACK dstein's scheme); scratch:
  * aq_tridiag_eigvecs_scratch_doubles() doubles.
  * aq_hetrd_vblocks / aq_hetrd_tfactors: the vectors v_i in blocks of nb <=
- * 64 (V [batch, nblk, nb, n], nblk = ceil((n-1)/nb), zero rows past i = n-2)
+ * 64, or an even nb <= 128 (two half-blocks' factors joined), (V [batch,
+ * nblk, nb, n], nblk = ceil((n-1)/nb), zero rows past i = n-2)
  * and each block's forward compact-WY factor T' (nb x nb, from the block's
  * Gram Gv = conj(V) V^T [batch, nblk, nb, nb]) with
  * H_{i0}^H ... H_{i0+nb-1}^H = I - V^T T' conj(V): the back-transformation
