@@ -385,8 +385,18 @@ def grouped_gemm_descs(descs: np.ndarray, dtype: torch.dtype, device, keep=None,
     them until the stream has run) or synchronised here.  ``rows32``: 32-row
     tiles of 128 threads (an AQ_GROUPED_HERK problem is then a full
     product)."""
+    launch = grouped_gemm_prepare(descs, dtype, device, keep, xcd_ranges, rows32)
+    if launch is not None:
+        launch()
+
+
+def grouped_gemm_prepare(descs: np.ndarray, dtype: torch.dtype, device, keep=None,
+                         xcd_ranges: bool = False, rows32: bool = False):
+    """``grouped_gemm_descs`` in two steps: the uploads now (they synchronise
+    the current stream), the launch when the returned function is called (on
+    the stream current then); None for an empty list."""
     if len(descs) == 0:
-        return
+        return None
     ka = bool(np.all(descs["a_s1"] == 1))
     nb = bool(np.all(descs["b_s1"] == 1))
     if not ka and np.any(descs["a_s1"] == 1) or not nb and np.any(descs["b_s1"] == 1):
@@ -407,18 +417,20 @@ def grouped_gemm_descs(descs: np.ndarray, dtype: torch.dtype, device, keep=None,
     grid = int(min(total, 65536))
     if xcd_ranges and grid >= 8:
         grid -= grid % 8          # XCD-contiguous tile ranges need a multiple of 8
-    st = torch.cuda.current_stream(device)
     no_theta = not bool(np.any(descs["kind"] == _ffi.AQ_GROUPED_THETA))
-    check(lib().aq_grouped_gemm(_code(dtype), ctypes.c_void_p(dd.data_ptr()),
-                                ctypes.c_void_p(pd.data_ptr()), len(descs), total,
-                                int(ka) | (int(nb) << 1) | (int(no_theta) << 2)
-                                | (int(bool(xcd_ranges)) << 3)
-                                | (_ffi.AQ_GROUPED_LAYOUT_ROWS32 if rows32 else 0), grid,
-                                ctypes.c_void_p(st.cuda_stream)), "aq_grouped_gemm")
-    if keep is None:
-        st.synchronize()
-    else:
+    layout = (int(ka) | (int(nb) << 1) | (int(no_theta) << 2) | (int(bool(xcd_ranges)) << 3)
+              | (_ffi.AQ_GROUPED_LAYOUT_ROWS32 if rows32 else 0))
+    if keep is not None:
         keep.extend([dd, pd])
+
+    def launch():
+        st = torch.cuda.current_stream(device)
+        check(lib().aq_grouped_gemm(_code(dtype), ctypes.c_void_p(dd.data_ptr()),
+                                    ctypes.c_void_p(pd.data_ptr()), len(descs), total, layout,
+                                    grid, ctypes.c_void_p(st.cuda_stream)), "aq_grouped_gemm")
+        if keep is None:
+            st.synchronize()
+    return launch
 
 
 def grouped_gemm(problems, dtype: torch.dtype, device) -> None:

@@ -663,24 +663,13 @@ struct HetrdGroup {
 };
 }  // namespace aq
 
-extern "C" int aq_hetrd_own_groups(int64_t n, int64_t nb, void* Av, int64_t strideA,
-                                   double* D, double* E, void* tauv, void* ws, int64_t ws_elems,
-                                   int64_t batch, int groups, hipStream_t stream) {
-  AQ_REQUIRE(Av && D && E && tauv && ws, AQ_ERR_ARG, "aq_hetrd_own: null pointer");
-  AQ_REQUIRE(groups >= 1 && groups <= kHetrdStreams, AQ_ERR_ARG,
-             "aq_hetrd_own: groups %d out of range", groups);
-  AQ_REQUIRE(n >= 2 && n <= 8192 && nb >= 1 && nb <= 32, AQ_ERR_ARG,
-             "aq_hetrd_own: n %lld / nb %lld out of range", (long long)n, (long long)nb);
-  AQ_REQUIRE(batch >= 1 && batch <= 65535 && strideA >= n * n, AQ_ERR_ARG,
-             "aq_hetrd_own: bad batch or stride");
-  AQ_REQUIRE(ws_elems >= aq_hetrd_workspace_elems(n, nb, batch), AQ_ERR_ARG,
-             "aq_hetrd_own: workspace too small");
-  if (n <= kSmallN) {
-    hetrd_small_kernel<<<(unsigned)batch, 1024, 0, stream>>>((double2*)Av, strideA, (int)n, D, E,
-                                                             (double2*)tauv);
-    AQ_LAUNCH_CHECK();
-    return AQ_OK;
-  }
+namespace aq {
+// The column loop over up to kHetrdStreams groups of consecutive matrices,
+// group q (cnt[q] matrices) on stream st[q], their launches interleaved
+// column by column.
+static int hetrd_columns(int64_t n, int64_t nb, void* Av, int64_t strideA, double* D, double* E,
+                         void* tauv, void* ws, int64_t batch, int ngroups, const int64_t* cnt,
+                         const hipStream_t* streams) {
   const int64_t nblk_max = cdiv(n, kHvTile);
   const int64_t sW = 3 * nb * n, sPart = 2 * nblk_max * n, sDots = 64 * nblk_max;
   const int64_t nred_max = cdiv(n, kWide), tiles_max = nblk_max * (nblk_max + 1) / 2;
@@ -689,21 +678,9 @@ extern "C" int aq_hetrd_own_groups(int64_t n, int64_t nb, void* Av, int64_t stri
   double2* part0 = VWV0 + batch * sW;
   double2* dots0 = part0 + batch * sPart;
   double* r0 = (double*)(dots0 + batch * sDots);
-  // The batch runs as up to two groups on worker streams, their column steps
-  // interleaved: one group's small per-matrix step overlaps the other's
-  // streaming matrix-vector launch.
-  // (from two matrices on: at small batches every launch is latency-bound,
-  // and the second group's chain fills the first one's gaps)
-  const int ngroups = batch >= 2 ? groups : 1;
-  hipStream_t ws_streams[kHetrdStreams];
   HetrdGroup g[kHetrdStreams];
-  if (ngroups > 1) {
-    const int st = hetrd_streams(ws_streams);
-    if (st != AQ_OK) return st;
-  }
   int64_t start = 0;
   for (int q = 0; q < ngroups; ++q) {
-    const int64_t cnt = batch / ngroups + (q < batch % ngroups ? 1 : 0);
     g[q].A = (double2*)Av + start * strideA;
     g[q].VWV = VWV0 + start * sW;
     g[q].W = g[q].VWV + nb * n;
@@ -714,15 +691,9 @@ extern "C" int aq_hetrd_own_groups(int64_t n, int64_t nb, void* Av, int64_t stri
     g[q].tau = (double2*)tauv + start * (n - 1);
     g[q].D = D + start * n;
     g[q].E = E + start * (n - 1);
-    g[q].batch = cnt;
-    g[q].s = ngroups > 1 ? ws_streams[q] : stream;
-    start += cnt;
-  }
-  hipEvent_t ready = nullptr;
-  if (ngroups > 1) {
-    AQ_HIP(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
-    AQ_HIP(hipEventRecord(ready, stream));
-    for (int q = 0; q < ngroups; ++q) AQ_HIP(hipStreamWaitEvent(g[q].s, ready, 0));
+    g[q].batch = cnt[q];
+    g[q].s = streams[q];
+    start += cnt[q];
   }
   int64_t N = n;
   for (; N > 1; N -= nb) {
@@ -787,9 +758,54 @@ extern "C" int aq_hetrd_own_groups(int64_t n, int64_t nb, void* Av, int64_t stri
       AQ_LAUNCH_CHECK();
     }
   }
+  return AQ_OK;
+}
+}  // namespace aq
+
+extern "C" int aq_hetrd_own_groups(int64_t n, int64_t nb, void* Av, int64_t strideA,
+                                   double* D, double* E, void* tauv, void* ws, int64_t ws_elems,
+                                   int64_t batch, int groups, hipStream_t stream) {
+  AQ_REQUIRE(Av && D && E && tauv && ws, AQ_ERR_ARG, "aq_hetrd_own: null pointer");
+  AQ_REQUIRE(groups >= 1 && groups <= kHetrdStreams, AQ_ERR_ARG,
+             "aq_hetrd_own: groups %d out of range", groups);
+  AQ_REQUIRE(n >= 2 && n <= 8192 && nb >= 1 && nb <= 32, AQ_ERR_ARG,
+             "aq_hetrd_own: n %lld / nb %lld out of range", (long long)n, (long long)nb);
+  AQ_REQUIRE(batch >= 1 && batch <= 65535 && strideA >= n * n, AQ_ERR_ARG,
+             "aq_hetrd_own: bad batch or stride");
+  AQ_REQUIRE(ws_elems >= aq_hetrd_workspace_elems(n, nb, batch), AQ_ERR_ARG,
+             "aq_hetrd_own: workspace too small");
+  if (n <= kSmallN) {
+    hetrd_small_kernel<<<(unsigned)batch, 1024, 0, stream>>>((double2*)Av, strideA, (int)n, D, E,
+                                                             (double2*)tauv);
+    AQ_LAUNCH_CHECK();
+    return AQ_OK;
+  }
+  // The batch runs as up to two groups on worker streams, their column steps
+  // interleaved: one group's small per-matrix step overlaps the other's
+  // streaming matrix-vector launch.
+  // (from two matrices on: at small batches every launch is latency-bound,
+  // and the second group's chain fills the first one's gaps)
+  const int ngroups = batch >= 2 ? groups : 1;
+  hipStream_t st[kHetrdStreams];
+  int64_t cnt[kHetrdStreams];
+  if (ngroups > 1) {
+    const int rc = hetrd_streams(st);
+    if (rc != AQ_OK) return rc;
+  } else {
+    st[0] = stream;
+  }
+  for (int q = 0; q < ngroups; ++q) cnt[q] = batch / ngroups + (q < batch % ngroups ? 1 : 0);
+  hipEvent_t ready = nullptr;
+  if (ngroups > 1) {
+    AQ_HIP(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+    AQ_HIP(hipEventRecord(ready, stream));
+    for (int q = 0; q < ngroups; ++q) AQ_HIP(hipStreamWaitEvent(st[q], ready, 0));
+  }
+  const int rc = hetrd_columns(n, nb, Av, strideA, D, E, tauv, ws, batch, ngroups, cnt, st);
+  if (rc != AQ_OK) return rc;
   if (ngroups > 1) {
     for (int q = 0; q < ngroups; ++q) {
-      AQ_HIP(hipEventRecord(ready, g[q].s));
+      AQ_HIP(hipEventRecord(ready, st[q]));
       AQ_HIP(hipStreamWaitEvent(stream, ready, 0));
     }
     AQ_HIP(hipEventDestroy(ready));
