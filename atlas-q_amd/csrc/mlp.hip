@@ -7,13 +7,18 @@
 // Backward from dy (dL/dy per sample): gradients of all weights / biases,
 // reduced deterministically (per-block partials, then a fixed-order sum).
 //
-// One work-group of 128 threads handles a tile of 32 samples; thread j owns
-// hidden unit j.  The BF16 weights are converted to FP32 into LDS once per
-// work-group.  Layer sizes are fixed by the model, so every index is a
+// One work-group of 128 threads (two waves) handles a tile of 32 samples.  The
+// layer products run on v_mfma_f32_32x32x16_bf16 (32 x 32 output blocks, 16 k
+// per instruction, FP32 accumulation): the weights enter as their BF16
+// copies, exactly; an FP32 activation or gradient operand is split into
+// BF16 hi + lo (x = hi + lo to 2^-16 relative), so a product with a weight is
+// two MFMAs and a weight gradient (both operands FP32) the three-product
+// BF16x3 form.  Layer sizes are fixed by the model, so every index is a
 // compile-time bound.
 #include <math.h>
 
 #include "aq_common.h"
+#include "bf16x3.h"
 
 namespace aq {
 
@@ -37,47 +42,116 @@ __device__ __forceinline__ float gelu_grad(float z) {
 }
 __device__ __forceinline__ float bf16r(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
+// LDS of a tile.  Row-major planes with k contiguous and 8 elements of row
+// padding (16-byte aligned rows for the MFMA operand loads).
+constexpr int kLdIn = kIn + 8;
+constexpr int kLdH = kH + 8;
+
 struct MlpSmem {
-  float w1[kH][kIn + 1];
-  float w2[kH][kH + 1];
+  uint16_t w1[kH][kLdIn];          // W1 (BF16): [unit][input]
+  uint16_t w2[kH][kLdH];           // W2 (BF16): [output unit][input unit]
+  uint16_t x[2][kTile][kLdIn];     // X hi / lo: [sample][input]
+  uint16_t h1[2][kTile][kLdH];     // gelu(z1) hi / lo: [sample][unit]
+  float z1[kTile][kH + 1];
+  float z2[kTile][kH + 1];         // the backward pass overwrites it with dL/dz2
   float w3[kH];
-  float x[kTile][kIn];
-  float z1[kTile][kH];
-  float h1[kTile][kH];
-  float z2[kTile][kH];
+  float dys[kTile];
+};
+struct MlpBwdSmem {
+  MlpSmem f;
+  uint16_t d2[2][kTile][kLdH];     // dL/dz2 hi / lo: [sample][unit]
+  uint16_t d1[2][kTile][kLdH];     // dL/dz1 hi / lo: [sample][unit]
 };
 
-// weights: bf16 copies (W1|W2|W3 in the same layout as the master, biases FP32)
-__device__ void load_weights(MlpSmem& sm, const uint16_t* __restrict__ wb) {
-  for (int i = threadIdx.x; i < kH * kIn; i += blockDim.x) sm.w1[i / kIn][i % kIn] = bf16r(wb[oW1 + i]);
-  for (int i = threadIdx.x; i < kH * kH; i += blockDim.x) sm.w2[i / kH][i % kH] = bf16r(wb[oW2 + i]);
-  for (int i = threadIdx.x; i < kH; i += blockDim.x) sm.w3[i] = bf16r(wb[oW3 + i]);
+// MFMA operand fragments: lane (r = lane & 31, q = lane >> 5) holds 8
+// consecutive k (k0 + 8q .. k0 + 8q + 7) of row / column r.
+__device__ __forceinline__ bf16x8 frag_rows(const uint16_t* base, int ld, int k0) {
+  const int lane = threadIdx.x & 63;
+  return load8(base + (lane & 31) * ld + k0 + 8 * (lane >> 5));
+}
+// the same from a [k][r] layout (k strided by ld)
+__device__ __forceinline__ bf16x8 frag_cols(const uint16_t* base, int ld, int k0) {
+  const int lane = threadIdx.x & 63;
+  const uint16_t* p = base + (int64_t)(k0 + 8 * (lane >> 5)) * ld + (lane & 31);
+  i16x8 v;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) v[e] = (short)p[e * ld];
+  return __builtin_bit_cast(bf16x8, v);
+}
+// accumulator register r of this lane -> (row, column) of the 32 x 32 block
+__device__ __forceinline__ int acc_row(int r) {
+  return 8 * (r >> 2) + 4 * ((threadIdx.x & 63) >> 5) + (r & 3);
+}
+__device__ __forceinline__ int acc_col() { return threadIdx.x & 31; }
+
+__device__ __forceinline__ void put_split(uint16_t (*pl)[kTile][kLdH], int s, int j, float v) {
+  uint16_t h, l;
+  split_bf16(v, h, l);
+  pl[0][s][j] = h;
+  pl[1][s][j] = l;
 }
 
-// forward of one tile; leaves z1, z2 in LDS and returns y for samples s < cnt
+// weights (BF16 copies) and the tile's inputs into LDS
+__device__ void load_tile(MlpSmem& sm, const uint16_t* __restrict__ wb,
+                          const float* __restrict__ X, int64_t s0, int cnt) {
+  for (int i = threadIdx.x; i < kH * kIn; i += blockDim.x) sm.w1[i / kIn][i % kIn] = wb[oW1 + i];
+  for (int i = threadIdx.x; i < kH * kH; i += blockDim.x) sm.w2[i / kH][i % kH] = wb[oW2 + i];
+  for (int i = threadIdx.x; i < kH; i += blockDim.x) sm.w3[i] = bf16r(wb[oW3 + i]);
+  for (int i = threadIdx.x; i < kTile * kIn; i += blockDim.x) {
+    const int s = i / kIn, k = i % kIn;
+    uint16_t h, l;
+    split_bf16(s < cnt ? X[(s0 + s) * kIn + k] : 0.f, h, l);
+    sm.x[0][s][k] = h;
+    sm.x[1][s][k] = l;
+  }
+}
+
+// forward of one tile on MFMA (wave w computes units 64 w .. 64 w + 63);
+// leaves z1, h1, z2 in LDS and returns y for samples s < cnt
 __device__ void tile_forward(MlpSmem& sm, const float* __restrict__ bias, int cnt,
                              float* __restrict__ y_out, int64_t s0) {
-  const int j = threadIdx.x;
-  const float b1 = bias[oB1 + j], b2 = bias[oB2 + j];
-  for (int s = 0; s < cnt; ++s) {
-    float a = b1;
+  const int wave = threadIdx.x >> 6;
+  // layer 1: z1 = X W1^T + b1 (K = 32)
+  for (int jb = 2 * wave; jb < 2 * wave + 2; ++jb) {
+    f32x16 acc = {};
 #pragma unroll
-    for (int k = 0; k < kIn; ++k) a += sm.w1[j][k] * sm.x[s][k];
-    sm.z1[s][j] = a;
-    sm.h1[s][j] = gelu_f(a);
+    for (int k0 = 0; k0 < kIn; k0 += 16) {
+      const bf16x8 b = frag_rows(&sm.w1[32 * jb][0], kLdIn, k0);
+      acc = mma_bf16(frag_rows(&sm.x[1][0][0], kLdIn, k0), b, acc);
+      acc = mma_bf16(frag_rows(&sm.x[0][0][0], kLdIn, k0), b, acc);
+    }
+    const int j = 32 * jb + acc_col();
+    const float b1 = bias[oB1 + j];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int s = acc_row(r);
+      const float z = acc[r] + b1;
+      sm.z1[s][j] = z;
+      put_split(sm.h1, s, j, gelu_f(z));
+    }
   }
   __syncthreads();
-  for (int s = 0; s < cnt; ++s) {
-    float a = b2;
-    for (int k = 0; k < kH; ++k) a += sm.w2[j][k] * sm.h1[s][k];
-    sm.z2[s][j] = a;
+  // layer 2: z2 = H1 W2^T + b2 (K = 128)
+  for (int ib = 2 * wave; ib < 2 * wave + 2; ++ib) {
+    f32x16 acc = {};
+#pragma unroll
+    for (int k0 = 0; k0 < kH; k0 += 16) {
+      const bf16x8 b = frag_rows(&sm.w2[32 * ib][0], kLdH, k0);
+      acc = mma_bf16(frag_rows(&sm.h1[1][0][0], kLdH, k0), b, acc);
+      acc = mma_bf16(frag_rows(&sm.h1[0][0][0], kLdH, k0), b, acc);
+    }
+    const int i = 32 * ib + acc_col();
+    const float b2 = bias[oB2 + i];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) sm.z2[acc_row(r)][i] = acc[r] + b2;
   }
   __syncthreads();
   // y = w3 . gelu(z2) + b3: thread s < cnt reduces sample s (fixed order)
-  if (j < cnt && y_out) {
+  const int t = threadIdx.x;
+  if (t < cnt && y_out) {
     float a = bias[oB3];
-    for (int k = 0; k < kH; ++k) a += sm.w3[k] * gelu_f(sm.z2[j][k]);
-    y_out[s0 + j] = a;
+    for (int k = 0; k < kH; ++k) a += sm.w3[k] * gelu_f(sm.z2[t][k]);
+    y_out[s0 + t] = a;
   }
 }
 
@@ -85,15 +159,11 @@ __global__ __launch_bounds__(128) void mlp_forward_kernel(const float* __restric
                                                           const uint16_t* __restrict__ wb,
                                                           const float* __restrict__ master,
                                                           float* __restrict__ y) {
-  extern __shared__ unsigned char smraw[];
+  extern __shared__ __align__(16) unsigned char smraw[];
   MlpSmem& sm = *reinterpret_cast<MlpSmem*>(smraw);
-  load_weights(sm, wb);
   const int64_t s0 = (int64_t)blockIdx.x * kTile;
   const int cnt = (int)((B - s0) < kTile ? (B - s0) : kTile);
-  for (int i = threadIdx.x; i < kTile * kIn; i += blockDim.x) {
-    const int s = i / kIn, k = i % kIn;
-    sm.x[s][k] = (s < cnt) ? X[(s0 + s) * kIn + k] : 0.f;
-  }
+  load_tile(sm, wb, X, s0, cnt);
   __syncthreads();
   tile_forward(sm, master, cnt, y, s0);
 }
@@ -104,57 +174,89 @@ __global__ __launch_bounds__(128) void mlp_backward_kernel(const float* __restri
                                                            int64_t B, const uint16_t* __restrict__ wb,
                                                            const float* __restrict__ master,
                                                            float* __restrict__ part) {
-  extern __shared__ unsigned char smraw[];
-  MlpSmem& sm = *reinterpret_cast<MlpSmem*>(smraw);
-  __shared__ float d2[kTile][kH];   // dL/dz2
-  __shared__ float dys[kTile];
-  load_weights(sm, wb);
+  extern __shared__ __align__(16) unsigned char smraw[];
+  MlpBwdSmem& bs = *reinterpret_cast<MlpBwdSmem*>(smraw);
+  MlpSmem& sm = bs.f;
   const int64_t s0 = (int64_t)blockIdx.x * kTile;
   const int cnt = (int)((B - s0) < kTile ? (B - s0) : kTile);
-  for (int i = threadIdx.x; i < kTile * kIn; i += blockDim.x) {
-    const int s = i / kIn, k = i % kIn;
-    sm.x[s][k] = (s < cnt) ? X[(s0 + s) * kIn + k] : 0.f;
-  }
-  if (threadIdx.x < kTile) dys[threadIdx.x] = (threadIdx.x < cnt) ? dy[s0 + threadIdx.x] : 0.f;
+  load_tile(sm, wb, X, s0, cnt);
+  if (threadIdx.x < kTile) sm.dys[threadIdx.x] = (threadIdx.x < cnt) ? dy[s0 + threadIdx.x] : 0.f;
   __syncthreads();
   tile_forward(sm, master, cnt, nullptr, s0);
-  const int j = threadIdx.x;
+  const int j = threadIdx.x;            // unit j of the element-wise steps
+  const int wave = threadIdx.x >> 6;
   float* P = part + (int64_t)blockIdx.x * kNParam;
-  // layer 3
-  float gw3 = 0.f, gb3 = 0.f;
-  for (int s = 0; s < cnt; ++s) {
-    const float z2 = sm.z2[s][j];
-    gw3 += dys[s] * gelu_f(z2);
-    d2[s][j] = dys[s] * sm.w3[j] * gelu_grad(z2);
-    gb3 += dys[s];
+  // layer 3 and dL/dz2 (rows past cnt zero)
+  float gw3 = 0.f, gb3 = 0.f, gb2 = 0.f;
+  for (int s = 0; s < kTile; ++s) {
+    float d = 0.f;
+    if (s < cnt) {
+      const float z2 = sm.z2[s][j];
+      gw3 += sm.dys[s] * gelu_f(z2);
+      d = sm.dys[s] * sm.w3[j] * gelu_grad(z2);
+      gb3 += sm.dys[s];
+    }
+    gb2 += d;
+    put_split(bs.d2, s, j, d);
   }
   P[oW3 + j] = gw3;
+  P[oB2 + j] = gb2;
   if (j == 0) P[oB3] = gb3;
   __syncthreads();
-  // layer 2: dW2[j][k] = sum_s d2[s][j] h1[s][k]; thread j owns row j
-  float gb2 = 0.f;
-  for (int s = 0; s < cnt; ++s) gb2 += d2[s][j];
-  P[oB2 + j] = gb2;
-  for (int k = 0; k < kH; ++k) {
-    float a = 0.f;
-    for (int s = 0; s < cnt; ++s) a += d2[s][j] * sm.h1[s][k];
-    P[oW2 + j * kH + k] = a;
+  // layer 2 weights: dW2[i][k] = sum_s d2[s][i] h1[s][k] (K = 32 samples, BF16x3)
+  for (int blk = wave; blk < 16; blk += 2) {
+    const int ib = blk >> 2, kb = blk & 3;
+    f32x16 acc = {};
+#pragma unroll
+    for (int k0 = 0; k0 < kTile; k0 += 16) {
+      const bf16x8 ah = frag_cols(&bs.d2[0][0][32 * ib], kLdH, k0);
+      const bf16x8 al = frag_cols(&bs.d2[1][0][32 * ib], kLdH, k0);
+      const bf16x8 bh = frag_cols(&sm.h1[0][0][32 * kb], kLdH, k0);
+      const bf16x8 bl = frag_cols(&sm.h1[1][0][32 * kb], kLdH, k0);
+      acc = mma_bf16(ah, bl, acc);
+      acc = mma_bf16(al, bh, acc);
+      acc = mma_bf16(ah, bh, acc);
+    }
+#pragma unroll
+    for (int r = 0; r < 16; ++r)
+      P[oW2 + (32 * ib + acc_row(r)) * kH + 32 * kb + acc_col()] = acc[r];
   }
-  // dL/dz1[s][j] = gelu'(z1[s][j]) * sum_i d2[s][i] W2[i][j]  (thread j = unit j)
-  float d1[kTile];
-  for (int s = 0; s < kTile; ++s) {
-    float a = 0.f;
-    if (s < cnt)
-      for (int i = 0; i < kH; ++i) a += d2[s][i] * sm.w2[i][j];
-    d1[s] = (s < cnt) ? a * gelu_grad(sm.z1[s][j]) : 0.f;
+  // dL/dz1[s][k] = gelu'(z1[s][k]) sum_i d2[s][i] W2[i][k] (K = 128 units)
+  for (int kb = 2 * wave; kb < 2 * wave + 2; ++kb) {
+    f32x16 acc = {};
+#pragma unroll
+    for (int k0 = 0; k0 < kH; k0 += 16) {
+      const bf16x8 b = frag_cols(&sm.w2[0][32 * kb], kLdH, k0);
+      acc = mma_bf16(frag_rows(&bs.d2[1][0][0], kLdH, k0), b, acc);
+      acc = mma_bf16(frag_rows(&bs.d2[0][0][0], kLdH, k0), b, acc);
+    }
+    const int k = 32 * kb + acc_col();
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int s = acc_row(r);
+      put_split(bs.d1, s, k, s < cnt ? acc[r] * gelu_grad(sm.z1[s][k]) : 0.f);
+    }
   }
+  __syncthreads();
   float gb1 = 0.f;
-  for (int s = 0; s < cnt; ++s) gb1 += d1[s];
+  for (int s = 0; s < kTile; ++s) gb1 += bf16_to_f32(bs.d1[0][s][j]) + bf16_to_f32(bs.d1[1][s][j]);
   P[oB1 + j] = gb1;
-  for (int k = 0; k < kIn; ++k) {
-    float a = 0.f;
-    for (int s = 0; s < cnt; ++s) a += d1[s] * sm.x[s][k];
-    P[oW1 + j * kIn + k] = a;
+  // layer 1 weights: dW1[k][c] = sum_s d1[s][k] x[s][c] (K = 32 samples, BF16x3)
+  for (int kb = 2 * wave; kb < 2 * wave + 2; ++kb) {
+    f32x16 acc = {};
+#pragma unroll
+    for (int k0 = 0; k0 < kTile; k0 += 16) {
+      const bf16x8 ah = frag_cols(&bs.d1[0][0][32 * kb], kLdH, k0);
+      const bf16x8 al = frag_cols(&bs.d1[1][0][32 * kb], kLdH, k0);
+      const bf16x8 bh = frag_cols(&sm.x[0][0][0], kLdIn, k0);
+      const bf16x8 bl = frag_cols(&sm.x[1][0][0], kLdIn, k0);
+      acc = mma_bf16(ah, bl, acc);
+      acc = mma_bf16(al, bh, acc);
+      acc = mma_bf16(ah, bh, acc);
+    }
+#pragma unroll
+    for (int r = 0; r < 16; ++r)
+      P[oW1 + (32 * kb + acc_row(r)) * kIn + acc_col()] = acc[r];
   }
 }
 
@@ -208,7 +310,7 @@ static int set_attrs() {
   AQ_HIP(hipFuncSetAttribute((const void*)mlp_forward_kernel,
                              hipFuncAttributeMaxDynamicSharedMemorySize, sizeof(MlpSmem)));
   AQ_HIP(hipFuncSetAttribute((const void*)mlp_backward_kernel,
-                             hipFuncAttributeMaxDynamicSharedMemorySize, sizeof(MlpSmem)));
+                             hipFuncAttributeMaxDynamicSharedMemorySize, sizeof(MlpBwdSmem)));
   g_attr_set = true;
   return AQ_OK;
 }
@@ -252,8 +354,8 @@ extern "C" int aq_mlp_backward(const float* X, const float* dy, int64_t batch, c
   AQ_REQUIRE(workspace_floats >= nb * kNParam, AQ_ERR_ARG, "aq_mlp_backward: workspace too small");
   int rc = set_attrs();
   if (rc) return rc;
-  mlp_backward_kernel<<<(unsigned)nb, 128, sizeof(MlpSmem), stream>>>(X, dy, batch, wb, master,
-                                                                      workspace);
+  mlp_backward_kernel<<<(unsigned)nb, 128, sizeof(MlpBwdSmem), stream>>>(X, dy, batch, wb,
+                                                                         master, workspace);
   AQ_LAUNCH_CHECK();
   mlp_reduce_kernel<<<(kNParam + 255) / 256, 256, 0, stream>>>(workspace, nb, grad);
   AQ_LAUNCH_CHECK();

@@ -378,7 +378,10 @@ int aq_ungqr(int64_t m, int64_t n, const void* A, int64_t strideA, const void* T
  * Parameters: one FP32 master vector of aq_mlp_num_params() floats laid out
  * W1[128][32] | b1[128] | W2[128][128] | b2[128] | W3[128] | b3, and a BF16
  * copy (uint16) of the same length used by forward/backward for the weights.
- * X: float32 [batch, 32] (row-major), y: float32 [batch]. */
+ * X: float32 [batch, 32] (row-major), y: float32 [batch].  The layer products
+ * run on BF16 MFMA (v_mfma_f32_32x32x16_bf16, FP32 accumulation) with the
+ * BF16 weights exact and FP32 activations / gradients split into BF16 hi +
+ * lo (relative error ~2^-16 per product). */
 int aq_mlp_num_params(void);
 int aq_mlp_sync_bf16(const float* master, uint16_t* wb, hipStream_t stream);
 int aq_mlp_forward(const float* X, int64_t batch, const uint16_t* wb, const float* master,

@@ -55,6 +55,8 @@ def test_hip_mlp_forward_backward_match_oracle():
     dy = rng.normal(size=77).astype(np.float32)
     y_ref, _ = M.forward(p, X.astype(np.float64))
     y = rp.forward(torch.from_numpy(X).cuda()).cpu().numpy()
+    print(f"forward: max |y - y_oracle| = {np.abs(y - y_ref).max():.2e} "
+          f"(max |y_oracle| {np.abs(y_ref).max():.2e})", flush=True)
     np.testing.assert_allclose(y, y_ref, rtol=2e-4, atol=2e-4)
     from atlas_q._ffi import lib
     import ctypes
@@ -68,6 +70,8 @@ def test_hip_mlp_forward_backward_match_oracle():
     assert lib().aq_mlp_backward(P(Xd), P(dyd), 77, P(rp.wb), P(rp.master), P(g), P(ws), wsn, s) == 0
     g_ref = M.backward(p, X.astype(np.float64), dy.astype(np.float64))
     g = g.cpu().numpy()
+    print(f"backward: max |g - g_oracle| / max |g_oracle| = "
+          f"{np.abs(g - g_ref).max() / np.abs(g_ref).max():.2e}", flush=True)
     np.testing.assert_allclose(g, g_ref, rtol=1e-3, atol=1e-3 * np.abs(g_ref).max())
 
 
