@@ -22,6 +22,7 @@ import torch
 
 CUS = 256
 CANDIDATES = {"gemm": [256, 512, 1024, 2048], "elem": [1024, 4096, 16384]}
+TILE_ROWS = (64, 32)   # candidate tile heights of the grouped GEMM
 DEFAULT_PATH = Path(os.environ.get("ATLASQ_AUTOTUNE_CACHE",
                                    Path(__file__).with_name("autotune_cache.json")))
 
@@ -95,6 +96,8 @@ class Autotuner:
             return 64, self.heuristic(key, base)
         best, best_ms = None, float("inf")
         for rows, total in sorted(totals.items(), reverse=True):
+            if rows not in TILE_ROWS:
+                continue
             for g in CANDIDATES["gemm"]:
                 g = max(1, min(g, total))
                 trial_fn(rows, g)  # warm

@@ -108,6 +108,8 @@ def eigh_desc(G: torch.Tensor):
 
 # ---------------------------------------------------------------- tridiagonal path
 NB_BACK = 128      # Householder vectors per block of the MFMA back-transformation
+NB_BACK_SMALL = 64   # ... below order NB_BACK_MIN_ORDER (fewer, shorter blocks pay less
+NB_BACK_MIN_ORDER = 1024   # for the wider factor join than they gain)
 HETRD_NB = 32      # panel width of the library's tridiagonal reduction
 REDUCTION = "own"  # "own": the library's kernels; "rocsolver": rocSOLVER's zhetrd
 ORTH_TILE = 64     # rows per diagonal tile of the orthogonality re-check
@@ -229,7 +231,7 @@ class GramTridiag:
                                    n, K._ptr(kv), kpad, K._ptr(Et), kpad * n, K._ptr(sc), nsc,
                                    B, st), "aq_tridiag_eigvecs")
         del sc
-        nb = NB_BACK
+        nb = NB_BACK if n >= NB_BACK_MIN_ORDER else NB_BACK_SMALL
         nblk = -(-(n - 1) // nb)
         V = torch.empty((B, nblk, nb, n), dtype=torch.complex128, device=dev)
         check(L.aq_hetrd_vblocks(n, nb, K._ptr(self.G), n * n, K._ptr(V), B, st),

@@ -738,35 +738,17 @@ __global__ void hetrd_tfactor_kernel(int n, int nb, int nblk, const double2* __r
   for (int c = 0; c < nb; ++c) Tf[base + (int64_t)a * ld + c] = t[a][c];
 }
 
-// Joins the two diagonal factors of a 2h x 2h block (h <= 64) into the
-// block's factor: T'_12 = -T'_11 Gv_12 T'_22, T'_21 = 0 (the column recursion
-// above, split at h).  One work-group of 256 threads per (matrix, block).
-__global__ __launch_bounds__(256) void hetrd_tmerge_kernel(int h, int nblk,
-                                                           const double2* __restrict__ Gv,
-                                                           double2* __restrict__ Tf) {
-  __shared__ double2 m[64][65];
-  const int64_t b = blockIdx.y;
-  const int p = blockIdx.x;
+// Zeroes the lower-left h x h block of every 2h x 2h factor (its scratch
+// during the join below).
+__global__ __launch_bounds__(256) void hetrd_tlow_zero_kernel(int h, int64_t nblocks,
+                                                              double2* __restrict__ Tf) {
   const int ld = 2 * h;
-  const int64_t base = (b * nblk + p) * (int64_t)ld * ld;
-  const double2* G = Gv + base;
-  double2* T = Tf + base;
-  // m = Gv_12 T'_22 (T'_22 upper triangular)
-  for (int e = threadIdx.x; e < h * h; e += blockDim.x) {
-    const int a = e / h, c = e % h;
-    double2 acc = make_double2(0.0, 0.0);
-    for (int l = 0; l <= c; ++l)
-      acc = cadd(acc, cmul(G[(int64_t)a * ld + h + l], T[(int64_t)(h + l) * ld + h + c]));
-    m[a][c] = acc;
-  }
-  __syncthreads();
-  // T'_12 = -T'_11 m (T'_11 upper triangular); T'_21 = 0
-  for (int e = threadIdx.x; e < h * h; e += blockDim.x) {
-    const int a = e / h, c = e % h;
-    double2 acc = make_double2(0.0, 0.0);
-    for (int l = a; l < h; ++l) acc = cadd(acc, cmul(T[(int64_t)a * ld + l], m[l][c]));
-    T[(int64_t)a * ld + h + c] = make_double2(-acc.x, -acc.y);
-    T[(int64_t)(h + a) * ld + c] = make_double2(0.0, 0.0);
+  const int64_t total = nblocks * h * h;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t blk = e / (h * h);
+    const int r = (int)(e % (h * h)) / h, c = (int)(e % h);
+    Tf[blk * ld * ld + (int64_t)(h + r) * ld + c] = make_double2(0.0, 0.0);
   }
 }
 
@@ -938,6 +920,12 @@ extern "C" int aq_hetrd_vblocks(int64_t n, int64_t nb, const void* A, int64_t st
   return AQ_OK;
 }
 
+extern "C" int aq_bgemm(int dtype, int64_t M, int64_t N, int64_t K, double alpha, const void* A,
+                        int64_t a_s0, int64_t a_s1, int64_t a_sb, int conj_a, const void* B,
+                        int64_t b_s0, int64_t b_s1, int64_t b_sb, int conj_b, double beta, void* C,
+                        int64_t c_s0, int64_t c_s1, int64_t c_sb, int64_t batch,
+                        hipStream_t stream);
+
 extern "C" int aq_hetrd_tfactors(int64_t n, int64_t nb, const void* tau, int64_t stau,
                                  const void* Gv, void* Tf, int64_t batch, hipStream_t stream) {
   AQ_REQUIRE(tau && Gv && Tf, AQ_ERR_ARG, "aq_hetrd_tfactors: null pointer");
@@ -962,8 +950,24 @@ extern "C" int aq_hetrd_tfactors(int64_t n, int64_t nb, const void* tau, int64_t
                                                          (const double2*)Gv, (double2*)Tf,
                                                          (int)nb);
   AQ_LAUNCH_CHECK();
-  dim3 gm((unsigned)nblk, (unsigned)batch);
-  hetrd_tmerge_kernel<<<gm, 256, 0, stream>>>(h, (int)nblk, (const double2*)Gv, (double2*)Tf);
+  // the join T'_12 = -T'_11 Gv_12 T'_22 (the column recursion split at h) on
+  // the batched MFMA GEMM, with the lower-left block as scratch for
+  // Gv_12 T'_22, then that block zeroed
+  const int64_t nbb = nb * nb, blocks = nblk * batch;
+  for (int64_t b0 = 0; b0 < blocks; b0 += 65535) {   // the GEMM's batch limit
+    const int64_t cnt = std::min<int64_t>(65535, blocks - b0);
+    const double2* G = (const double2*)Gv + b0 * nbb;
+    double2* T = (double2*)Tf + b0 * nbb;
+    int rc = aq_bgemm(AQ_C128, h, h, h, 1.0, G + h, nb, 1, nbb, 0, T + (int64_t)h * nb + h, nb,
+                      1, nbb, 0, 0.0, T + (int64_t)h * nb, nb, 1, nbb, cnt, stream);
+    if (rc != AQ_OK) return rc;
+    rc = aq_bgemm(AQ_C128, h, h, h, -1.0, T, nb, 1, nbb, 0, T + (int64_t)h * nb, nb, 1, nbb, 0,
+                  0.0, T + h, nb, 1, nbb, cnt, stream);
+    if (rc != AQ_OK) return rc;
+  }
+  double2* T = (double2*)Tf;
+  hetrd_tlow_zero_kernel<<<(unsigned)std::min<int64_t>(cdiv(blocks * h * h, 256), 65536), 256, 0,
+                           stream>>>(h, blocks, T);
   AQ_LAUNCH_CHECK();
   return AQ_OK;
 }

@@ -135,6 +135,8 @@ def parse():
                     help="only run the sampler measurement and print its JSON object")
     ap.add_argument("--theta-probe", type=int, default=0, metavar="N",
                     help="N BF16x3 theta calls at bond --chi (random operands), for profiling")
+    ap.add_argument("--tile-rows", default=None, metavar="R,R",
+                    help="grouped GEMM tile heights the autotuner may choose (default 64,32)")
     ap.add_argument("--gram-chunks", default=None, metavar="F,F,...",
                     help="shares of a sub-layer's Gram batch factored as overlapped chunks "
                          "(default: the library's setting; 1 = one batch)")
@@ -1047,6 +1049,9 @@ def main():
     from atlas_q import fastsvd
     if a.gram_chunks:
         fastsvd.GRAM_CHUNKS = tuple(float(f) for f in a.gram_chunks.split(","))
+    if a.tile_rows:
+        from atlas_q import autotune
+        autotune.TILE_ROWS = tuple(int(r) for r in a.tile_rows.split(","))
     if a.sampler_only:
         print(json.dumps({"sampler": sampler_bench(torch, a, dev)}), flush=True)
         return
