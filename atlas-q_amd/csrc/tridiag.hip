@@ -222,44 +222,84 @@ __global__ __launch_bounds__(kTriThreads) void tri_invit_kernel(
     dp = guard_pivot(d[i + 1] - lam - l * e[i], piv);
     AQ_SC(Ps, i + 1) = dp;
   }
+  // The loops below read this thread's factors back from scratch; one wave
+  // runs per SIMD here, so nothing hides a load's latency but other loads:
+  // each loop fetches kPf entries ahead of the dependent recurrence (the
+  // arithmetic and its order are unchanged).
+  constexpr int kPf = 8;
   // progressive factorisation T - lam = U- D- U-^T (bottom up) and the twist
   double dm = guard_pivot(d[t] - lam, piv);
   int r = t;
   double best = fabs(AQ_SC(Ps, t));
-  for (int i = t - 1; i >= s; --i) {
-    const double u = e[i] / dm;
-    AQ_SC(Us, i) = u;
-    dm = guard_pivot(d[i] - lam - u * e[i], piv);
-    const double g = fabs(AQ_SC(Ps, i) + dm - (d[i] - lam));
-    if (g < best) {
-      best = g;
-      r = i;
+  for (int i = t - 1; i >= s;) {
+    double pf[kPf];
+#pragma unroll
+    for (int u = 0; u < kPf; ++u) pf[u] = (i - u >= s) ? AQ_SC(Ps, i - u) : 0.0;
+#pragma unroll
+    for (int u = 0; u < kPf; ++u, --i) {
+      if (i < s) break;
+      const double uu = e[i] / dm;
+      AQ_SC(Us, i) = uu;
+      dm = guard_pivot(d[i] - lam - uu * e[i], piv);
+      const double g = fabs(pf[u] + dm - (d[i] - lam));
+      if (g < best) {
+        best = g;
+        r = i;
+      }
     }
   }
   // z_r = 1, z_i = -L_i z_{i+1} above, z_{i+1} = -U_i z_i below
   double nrm2 = 1.0, zi = 1.0;
-  for (int i = r - 1; i >= s; --i) {
-    zi = -AQ_SC(Ls, i) * zi;
-    nrm2 += zi * zi;
+  for (int i = r - 1; i >= s;) {
+    double pf[kPf];
+#pragma unroll
+    for (int u = 0; u < kPf; ++u) pf[u] = (i - u >= s) ? AQ_SC(Ls, i - u) : 0.0;
+#pragma unroll
+    for (int u = 0; u < kPf; ++u, --i) {
+      if (i < s) break;
+      zi = -pf[u] * zi;
+      nrm2 += zi * zi;
+    }
   }
   zi = 1.0;
-  for (int i = r; i < t; ++i) {
-    zi = -AQ_SC(Us, i) * zi;
-    nrm2 += zi * zi;
+  for (int i = r; i < t;) {
+    double pf[kPf];
+#pragma unroll
+    for (int u = 0; u < kPf; ++u) pf[u] = (i + u < t) ? AQ_SC(Us, i + u) : 0.0;
+#pragma unroll
+    for (int u = 0; u < kPf; ++u, ++i) {
+      if (i >= t) break;
+      zi = -pf[u] * zi;
+      nrm2 += zi * zi;
+    }
   }
   const double inv = 1.0 / sqrt(nrm2);
   for (int i = 0; i < s; ++i) z[i] = make_double2(0.0, 0.0);
   for (int i = t + 1; i < n; ++i) z[i] = make_double2(0.0, 0.0);
   z[r] = make_double2(inv, 0.0);
   zi = 1.0;
-  for (int i = r - 1; i >= s; --i) {
-    zi = -AQ_SC(Ls, i) * zi;
-    z[i] = make_double2(zi * inv, 0.0);
+  for (int i = r - 1; i >= s;) {
+    double pf[kPf];
+#pragma unroll
+    for (int u = 0; u < kPf; ++u) pf[u] = (i - u >= s) ? AQ_SC(Ls, i - u) : 0.0;
+#pragma unroll
+    for (int u = 0; u < kPf; ++u, --i) {
+      if (i < s) break;
+      zi = -pf[u] * zi;
+      z[i] = make_double2(zi * inv, 0.0);
+    }
   }
   zi = 1.0;
-  for (int i = r; i < t; ++i) {
-    zi = -AQ_SC(Us, i) * zi;
-    z[i + 1] = make_double2(zi * inv, 0.0);
+  for (int i = r; i < t;) {
+    double pf[kPf];
+#pragma unroll
+    for (int u = 0; u < kPf; ++u) pf[u] = (i + u < t) ? AQ_SC(Us, i + u) : 0.0;
+#pragma unroll
+    for (int u = 0; u < kPf; ++u, ++i) {
+      if (i >= t) break;
+      zi = -pf[u] * zi;
+      z[i + 1] = make_double2(zi * inv, 0.0);
+    }
   }
 #undef AQ_SC
 }
