@@ -364,6 +364,7 @@ __global__ __launch_bounds__(kClThreads) void tri_cluster_kernel(
     double2* __restrict__ Z, int64_t sZ, double* __restrict__ scratch, int64_t sScr) {
   __shared__ double d[NMAX], e2[NMAX], e[NMAX];
   __shared__ int nsplit, nroots;
+  __shared__ int s_rows[NMAX], s_nrows;
   const int64_t b = blockIdx.x;
   const TriBounds tb = load_tridiag(D + b * sD, E + b * sE, n, d, e2, e, true);
   const int kb = kvec ? min((int)kvec[b], kmax) : kmax;
@@ -475,12 +476,19 @@ __global__ __launch_bounds__(kClThreads) void tri_cluster_kernel(
   }
   __threadfence_block();
   __syncthreads();
-  // the members' rows of Z, written by the whole work-group (coalesced)
-  for (int64_t e0 = threadIdx.x; e0 < (int64_t)kb * n; e0 += blockDim.x) {
-    const int j = (int)(e0 / n), i = (int)(e0 % n);
-    if (bigroot[j] < 0) continue;
-    Zb[(int64_t)j * n + i] =
-        make_double2(i >= bs[j] && i <= bt[j] ? X[(int64_t)i * ld + j] : 0.0, 0.0);
+  // the members' rows of Z, written by the whole work-group (coalesced): the
+  // members listed in LDS first (any order: the rows are independent), so
+  // the work-group walks those rows only
+  if (threadIdx.x == 0) s_nrows = 0;
+  __syncthreads();
+  for (int j = threadIdx.x; j < kb; j += blockDim.x)
+    if (bigroot[j] >= 0) s_rows[atomicAdd(&s_nrows, 1)] = j;
+  __syncthreads();
+  const int nrows = s_nrows;
+  for (int q = 0; q < nrows; ++q) {
+    const int j = s_rows[q], s = bs[j], t = bt[j];
+    for (int i = threadIdx.x; i < n; i += blockDim.x)
+      Zb[(int64_t)j * n + i] = make_double2(i >= s && i <= t ? X[(int64_t)i * ld + j] : 0.0, 0.0);
   }
 }
 
