@@ -178,7 +178,7 @@ def test_bgemm_matches_float64_products():
     assert np.abs(got - want).max() <= 1e-12 * np.abs(want).max()
 
 
-@pytest.mark.parametrize("n", [2, 33, 64, 65, 130, 256, 257, 512])
+@pytest.mark.parametrize("n", [2, 33, 64, 65, 130, 144, 203, 256, 257, 512])
 def test_own_tridiagonal_reduction_matches_lapack(n):
     """The library's blocked reduction against LAPACK's zhetrd (scipy, host,
     float64) on the column-major view conj(G) with uplo = "U": the same
